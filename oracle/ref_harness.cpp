@@ -1,0 +1,354 @@
+// ref_harness.cpp -- drives sonicSolver's own hot-path code on
+// programmatically built Fields.  TEST INFRASTRUCTURE ONLY.
+//
+// Built by oracle/Makefile (target `ref`) from the reference sources where
+// they lie; the output goes to oracle/_ref/libsfref.so.  It is used to pin
+// the C restatement (sfcpu.c) and as the timed CPU baseline.  Nothing in
+// the shipped package links or loads it.
+//
+// What is the reference's code here: Field (SF_field.cpp), the boundary
+// applicator with linear zeroGradient (SF_applicator.h, SF_boundary.cpp,
+// SF_thermodynamicClosure.cpp, SF_zeroGradient.cpp), divDispatch /
+// computeAllFluxes / characteristicWENOFaceFlux (SF_div.h, SF_WENO.h),
+// weno5_core / teno5_core / buildCharacteristicMatrix (SF_WENO5.cpp,
+// SF_TENO5.cpp, SF_riemann.cpp), RusanovEOS::faceFlux / deltaT
+// (SF_rusanovEOS.cpp), validateEquationState (SF_equationState.h) and
+// Time::Explicit::begin / executeStage (SF_explicit.cpp).
+//
+// What this file supplies itself:
+//  - the explicit tableaux, restated from SF_builtinProviders.cpp:85-127,
+//    because that translation unit pulls the whole case compiler (and
+//    through it HYPRE) into the link;
+//  - uniform Cartesian metrics and named boundary sets (the values that
+//    Mesh::computeMetrics and the mesh generator produce on a uniform
+//    grid, SF_mesh.cpp:320-338, SF_meshGen.cpp:514-541);
+//  - periodic ghost fill and the HLLC flux, which the reference does not
+//    have (the HLLC flux is the oracle's, plugged into the reference's
+//    characteristicWENOFaceFlux through its FaceFlux parameter).
+#include "core/field/SF_field.h"
+#include "core/flux/SF_flux.h"
+#include "core/residual/SF_residual.h"
+#include "solver/discretization/convection/SF_div.h"
+#include "solver/discretization/convection/SF_WENO.h"
+#include "methods/numerics/Flux/SF_rusanovEOS.h"
+#include "methods/numerics/state/SF_equationState.h"
+#include "methods/numerics/structured/SF_residualAssembly.h"
+#include "models/physics/fluidStateModel/SF_factory.h"
+#include "solver/algorithm/time/SF_explicit.h"
+#include "solver/boundary/SF_applicator.h"
+
+#include "sfcpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace {
+
+using Model = SF::Physics::FluidStateModel::Model;
+
+struct Case {
+    sfo_desc d;
+    std::shared_ptr<const Model> eq;
+    SF::Field field;
+    SF::Boundary::Applicator bc;
+    std::vector<SF::SolverAlgorithm::PatchWorkspace> ws;
+};
+
+void configure_directions(const sfo_desc& d) {
+    SF::Math::resetActiveDirections();
+    for (int a = 0; a < 3; ++a)
+        if (!d.active[a]) SF::Math::deactivateDirection(a);
+}
+
+SF::FDM::BoundaryConfig boundary_config(const sfo_desc& d) {
+    SF::FDM::BoundaryConfig cfg;
+    static const char* names[3][1] = {{"Left"}, {"sideWalls"}, {"frontBack"}};
+    std::vector<std::string> sets;
+    if (d.active[0] && d.bc[0] == SFO_TRANSMISSIVE) {
+        sets.push_back("Left");
+        sets.push_back("Right");
+    }
+    if (d.active[1] && d.bc[1] == SFO_TRANSMISSIVE) sets.push_back(names[1][0]);
+    if (d.active[2] && d.bc[2] == SFO_TRANSMISSIVE) sets.push_back(names[2][0]);
+    for (const auto& s : sets) {
+        cfg.density.push_back({s, SF::ZERO_GRADIENT, 0.0});
+        cfg.velocity.push_back({s, SF::ZERO_GRADIENT, SF::Vector3()});
+        cfg.pressure.push_back({s, SF::ZERO_GRADIENT, 0.0});
+    }
+    return cfg;
+}
+
+std::unique_ptr<Case> make_case(const sfo_desc& d, const double* soa) {
+    auto c = std::make_unique<Case>();
+    c->d = d;
+    c->eq = SF::Physics::FluidStateModel::makeSingleFluidPerfectGas(
+        d.gamma, 287.05, 0.0, 0.72);
+    configure_directions(d);
+    SF::Field& f = c->field;
+    f.setStateModel(c->eq);
+    const int ng = 3;
+    f.setup(d.n[0], d.n[1], d.n[2], ng, 5);
+    const double jac = 1.0 / (d.d[0] * d.d[1] * d.d[2]);
+    for (int k = 0; k < f.MZ(); ++k)
+        for (int j = 0; j < f.MY(); ++j)
+            for (int i = 0; i < f.MX(); ++i) {
+                f.X(i, j, k) = (i - ng) * d.d[0];
+                f.Y(i, j, k) = (j - ng) * d.d[1];
+                f.Z(i, j, k) = (k - ng) * d.d[2];
+                f.Jac(i, j, k) = jac;
+                f.XiX(i, j, k) = 1.0 / d.d[0];
+                f.EtY(i, j, k) = 1.0 / d.d[1];
+                f.ZeZ(i, j, k) = 1.0 / d.d[2];
+            }
+    std::map<std::string, std::vector<int>> sets;
+    const int nx = d.n[0], ny = d.n[1], nz = d.n[2];
+    for (int k = ng; k < ng + nz; ++k)
+        for (int j = ng; j < ng + ny; ++j) {
+            sets["Left"].push_back(f.getIdx(ng, j, k));
+            sets["Right"].push_back(f.getIdx(ng + nx - 1, j, k));
+        }
+    for (int k = ng; k < ng + nz; ++k)
+        for (int i = ng; i < ng + nx; ++i) {
+            sets["sideWalls"].push_back(f.getIdx(i, ng, k));
+            sets["sideWalls"].push_back(f.getIdx(i, ng + ny - 1, k));
+        }
+    for (int j = ng; j < ng + ny; ++j)
+        for (int i = ng; i < ng + nx; ++i) {
+            sets["frontBack"].push_back(f.getIdx(i, j, ng));
+            sets["frontBack"].push_back(f.getIdx(i, j, ng + nz - 1));
+        }
+    for (auto& kv : sets) {
+        std::sort(kv.second.begin(), kv.second.end());
+        kv.second.erase(std::unique(kv.second.begin(), kv.second.end()),
+                        kv.second.end());
+    }
+    f.setBoundarySets(sets);
+    c->bc = SF::Boundary::Applicator(boundary_config(d));
+    const size_t N = (size_t)nx * ny * nz;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                for (int v = 0; v < 5; ++v)
+                    f(i + ng, j + ng, k + ng, v) =
+                        soa[v * N + ((size_t)k * ny + j) * nx + i];
+    f.invalidateThermodynamicCache();
+    c->ws.resize(1);
+    c->ws[0].ensureFor(f);
+    return c;
+}
+
+void periodic_fill(Case& c) {
+    SF::Field& f = c.field;
+    const int ng = f.NG();
+    for (int a = 0; a < 3; ++a) {
+        if (!c.d.active[a] || c.d.bc[a] != SFO_PERIODIC) continue;
+        const int n = c.d.n[a];
+        int lo[3] = {ng, ng, ng}, hi[3] = {ng + c.d.n[0], ng + c.d.n[1], ng + c.d.n[2]};
+        int p[3];
+        const int e1 = (a + 1) % 3, e2 = (a + 2) % 3;
+        for (p[e2] = lo[e2]; p[e2] < hi[e2]; ++p[e2])
+            for (p[e1] = lo[e1]; p[e1] < hi[e1]; ++p[e1])
+                for (int l = 0; l < ng; ++l) {
+                    int gl[3] = {p[0], p[1], p[2]}, sl[3] = {p[0], p[1], p[2]};
+                    int gh[3] = {p[0], p[1], p[2]}, sh[3] = {p[0], p[1], p[2]};
+                    gl[a] = l;          sl[a] = ng + ((l - ng) % n + n) % n;
+                    gh[a] = ng + n + l; sh[a] = ng + l % n;
+                    for (int v = 0; v < 5; ++v) {
+                        f(gl[0], gl[1], gl[2], v) = f(sl[0], sl[1], sl[2], v);
+                        f(gh[0], gh[1], gh[2], v) = f(sh[0], sh[1], sh[2], v);
+                    }
+                }
+    }
+    f.invalidateThermodynamicCache();
+}
+
+void prepare_boundary(Case& c) {
+    c.bc.apply(c.field);
+    periodic_fill(c);
+    c.field.invalidateThermodynamicCache();
+}
+
+void hllc_face_flux(const double qL[5], const double qR[5], const double n[3],
+                    double out[5], double gamma) {
+    if (sfo_hllc(qL, qR, n, gamma, out) != 0) {
+        std::cerr << "[SF FATAL] HLLC received an invalid state." << std::endl;
+        std::exit(1);
+    }
+}
+
+// ConservativeRHS per-patch assembly for one stage (SF_conservativeRHS.cpp
+// :137-188 -> Compressible::System::convection -> divDispatch).
+void assemble(Case& c) {
+    prepare_boundary(c);
+    auto& w = c.ws[0];
+    w.clear();
+    const double gamma = c.d.gamma;
+    const auto scheme = c.d.scheme == SFO_TENO5 ? SF::FDM::ConvectionScheme::TENO5
+                                                : SF::FDM::ConvectionScheme::WENO5;
+    if (c.d.flux == SFO_RUSANOV) {
+        SF::divDispatch(c.field, w.convectiveFlux, w.residual, scheme,
+                        SF::FDM::FluxSplitter::Rusanov, 0.0,
+                        SF::FDM::IBMBoundaryScheme::LowOrder, 0, gamma, *c.eq);
+        return;
+    }
+    auto run = [&](auto core) {
+        SF::Flux::computeAllFluxes<6>(
+            c.field, w.convectiveFlux,
+            [&](const auto& s, const auto& m, auto* fl, int, int, int, SF::Math::Dir) {
+                SF::Flux::characteristicWENOFaceFlux<6>(
+                    s, m, fl, core,
+                    [&](const double qL[5], const double qR[5],
+                        const double n[3], double out[5]) {
+                        hllc_face_flux(qL, qR, n, out, gamma);
+                    },
+                    gamma);
+            },
+            SF::Flux::LowOrderFlux::Method::FirstOrderRoe,
+            SF::FDM::IBMBoundaryScheme::LowOrder, 0, gamma, c.eq.get());
+    };
+    if (c.d.scheme == SFO_TENO5) run(SF::TENO5::teno5_core);
+    else run(SF::WENO5::weno5_core);
+    SF::Math::assembleConvectiveFluxResidual(c.field, w.convectiveFlux, w.residual);
+}
+
+SF::System::CompiledTimeRecipe time_recipe(int time) {
+    using SF::System::CompiledTimeRecipe;
+    using SF::System::ExplicitStageBackend;
+    using SF::FDM::TimeRecipeId;
+    if (time == SFO_EULER)
+        return CompiledTimeRecipe::fromMethod(
+            SF::FDM::builtInTimeRecipe(TimeRecipeId::ForwardEuler),
+            {{{0.0, 0.0, 1.0}}}, 1, ExplicitStageBackend::ForwardEuler);
+    if (time == SFO_SSPRK3)
+        return CompiledTimeRecipe::fromMethod(
+            SF::FDM::builtInTimeRecipe(TimeRecipeId::SSPRK3),
+            {{{0.0, 0.0, 1.0}, {1.0, 0.75, 0.25}, {0.5, 1.0 / 3.0, 2.0 / 3.0}}}, 3,
+            ExplicitStageBackend::SSPRK3);
+    return CompiledTimeRecipe::fromMethod(
+        SF::FDM::builtInTimeRecipe(TimeRecipeId::ClassicalRK4),
+        {{{0.0, 0.0, 0.5}, {0.5, 0.0, 0.5}, {0.5, 0.0, 1.0}, {1.0, 0.0, 0.0}}}, 4,
+        ExplicitStageBackend::ClassicalRK4, {{1.0, 2.0, 2.0, 1.0}}, 6.0);
+}
+
+void export_state(const Case& c, double* soa) {
+    const SF::Field& f = c.field;
+    const int ng = f.NG();
+    const int nx = c.d.n[0], ny = c.d.n[1], nz = c.d.n[2];
+    const size_t N = (size_t)nx * ny * nz;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                for (int v = 0; v < 5; ++v)
+                    soa[v * N + ((size_t)k * ny + j) * nx + i] =
+                        f(i + ng, j + ng, k + ng, v);
+}
+
+}  // namespace
+
+extern "C" {
+
+double sfref_weno5(const double v[5]) { return SF::WENO5::weno5_core(v); }
+double sfref_teno5(const double v[5]) { return SF::TENO5::teno5_core(v); }
+
+int sfref_char_matrix(const double q[5], const double n[3], double gamma,
+                      double L[25], double R[25]) {
+    SF::Riemann::buildCharacteristicMatrix(q, n, gamma, L, R);
+    return 0;
+}
+
+int sfref_rusanov(const double qL[5], const double qR[5], const double n[3],
+                  double gamma, double out[5]) {
+    auto eq = SF::Physics::FluidStateModel::makeSingleFluidPerfectGas(
+        gamma, 287.05, 0.0, 0.72);
+    SF::Numerics::RusanovEOS::faceFlux(*eq, qL, qR, {n[0], n[1], n[2]}, out, 5);
+    return 0;
+}
+
+int sfref_char_face_flux(const double st[30], const double cof[3], int scheme,
+                         int flux, double gamma, double out[5]) {
+    double stencil[6][5];
+    for (int s = 0; s < 6; ++s)
+        for (int v = 0; v < 5; ++v) stencil[s][v] = st[5 * s + v];
+    const double metrics[4] = {cof[0], cof[1], cof[2], 1.0};
+    auto eq = SF::Physics::FluidStateModel::makeSingleFluidPerfectGas(
+        gamma, 287.05, 0.0, 0.72);
+    auto face = [&](const double qL[5], const double qR[5], const double n[3],
+                    double o[5]) {
+        if (flux == SFO_HLLC) hllc_face_flux(qL, qR, n, o, gamma);
+        else SF::Numerics::RusanovEOS::faceFlux(*eq, qL, qR, {n[0], n[1], n[2]}, o, 5);
+    };
+    if (scheme == SFO_TENO5)
+        SF::Flux::characteristicWENOFaceFlux<6>(stencil, metrics, out,
+                                                SF::TENO5::teno5_core, face, gamma);
+    else
+        SF::Flux::characteristicWENOFaceFlux<6>(stencil, metrics, out,
+                                                SF::WENO5::weno5_core, face, gamma);
+    return 0;
+}
+
+// Residual of one state after boundary preparation, on the unique nodes.
+int sfref_residual(const sfo_desc* d, const double* soa_in, double* soa_out) {
+    auto c = make_case(*d, soa_in);
+    assemble(*c);
+    const SF::Field& f = c->field;
+    const int ng = f.NG();
+    const int nx = d->n[0], ny = d->n[1], nz = d->n[2];
+    const size_t N = (size_t)nx * ny * nz;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const bool solved = !f.isSolverBoundaryPoint(i + ng, j + ng, k + ng);
+                for (int v = 0; v < 5; ++v)
+                    soa_out[v * N + ((size_t)k * ny + j) * nx + i] = solved
+                        ? SF::Math::spatialResidual(f, c->ws[0].residual,
+                                                    i + ng, j + ng, k + ng, v)
+                        : 0.0;
+            }
+    return 0;
+}
+
+// flow.step.prepare -> flow.dt.compute -> flow.step.begin ->
+// explicit.stage.execute x stages -> flow.step.commit, repeated nsteps
+// times (SF_flowOperations.cpp:466-522,648-652).
+int sfref_run(const sfo_desc* d, double* soa_io, int nsteps, double cfl,
+              double max_dt, double* dt_seq) {
+    auto c = make_case(*d, soa_io);
+    SF::State::StateBundle state;
+    state.patches = {&c->field};
+    state.stateModel = c->eq;
+    state.time = 0.0;
+    const auto recipe = time_recipe(d->time);
+    for (int s = 0; s < nsteps; ++s) {
+        prepare_boundary(*c);
+        double dt = SF::Numerics::RusanovEOS::deltaT(c->field, *c->eq, cfl);
+        dt = std::min(dt, max_dt);
+        state.dt = dt;
+        if (dt_seq) dt_seq[s] = dt;
+        SF::Time::Explicit::Workspace workspace;
+        SF::Time::Explicit::begin(workspace, state.patches, state, recipe, nullptr);
+        for (int st = 0; st < recipe.stageCount(); ++st) {
+            SF::Time::Explicit::executeStage(
+                workspace, st, state.patches, c->ws, state, nullptr,
+                [&](const std::vector<SF::Field*>&,
+                    std::vector<SF::SolverAlgorithm::PatchWorkspace>&, double) {
+                    assemble(*c);
+                },
+                [&](const std::vector<SF::Field*>&) {},
+                [&](const std::vector<SF::Field*>& patches, const char* label) {
+                    for (SF::Field* p : patches)
+                        SF::Numerics::validateEquationState(*p, *c->eq, label);
+                });
+        }
+        prepare_boundary(*c);
+        state.time += dt;
+        ++state.step;
+    }
+    export_state(*c, soa_io);
+    return 0;
+}
+
+}  // extern "C"

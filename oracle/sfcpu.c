@@ -1,0 +1,773 @@
+/*
+ * sfcpu.c -- CPU FP64 restatement of sonicSolver's density-based
+ * single-fluid explicit hot path.  TEST INFRASTRUCTURE ONLY (the checker);
+ * see sfcpu.h.  Build: oracle/Makefile (gcc -O2 -ffp-contract=off, so the
+ * arithmetic is evaluated exactly in source order without FMA contraction).
+ *
+ * Citations are relative to the sonicSolver source tree.
+ */
+#include "sfcpu.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static double sq(double x) { return x * x; }
+
+/* ---------------------------------------------------------------------- */
+/* Scalar reconstructions                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* WENO5-JS, Jiang & Shu (1996).  Follows WENO5::weno5_core,
+ * src/methods/numerics/convection/SF_WENO5.cpp:42-62 (eps = 1e-6,
+ * linear weights 0.1/0.6/0.3, alpha = d/(eps+beta)^2). */
+double sfo_weno5_core(const double v[5], unsigned *err)
+{
+    for (int s = 0; s < 5; ++s)
+        if (!isfinite(v[s])) { *err |= SFO_ERR_WENO; return NAN; }
+    double p0 = (2.0 * v[0] - 7.0 * v[1] + 11.0 * v[2]) / 6.0;
+    double p1 = (-1.0 * v[1] + 5.0 * v[2] + 2.0 * v[3]) / 6.0;
+    double p2 = (2.0 * v[2] + 5.0 * v[3] - 1.0 * v[4]) / 6.0;
+    const double d0 = 0.1, d1 = 0.6, d2 = 0.3;
+    double b0 = 13.0 / 12.0 * sq(v[0] - 2.0 * v[1] + v[2])
+              + 0.25 * sq(v[0] - 4.0 * v[1] + 3.0 * v[2]);
+    double b1 = 13.0 / 12.0 * sq(v[1] - 2.0 * v[2] + v[3])
+              + 0.25 * sq(v[1] - v[3]);
+    double b2 = 13.0 / 12.0 * sq(v[2] - 2.0 * v[3] + v[4])
+              + 0.25 * sq(3.0 * v[2] - 4.0 * v[3] + v[4]);
+    const double eps = 1e-6;
+    double a0 = d0 / sq(eps + b0);
+    double a1 = d1 / sq(eps + b1);
+    double a2 = d2 / sq(eps + b2);
+    double sum = a0 + a1 + a2;
+    if (!isfinite(sum) || sum <= 0.0) { *err |= SFO_ERR_WENO; return NAN; }
+    return (a0 * p0 + a1 * p1 + a2 * p2) / sum;
+}
+
+/* TENO5, Fu, Hu & Adams (2016) with C = 1, q = 6, CT = 1e-5.  Follows
+ * TENO5::teno5_core, src/methods/numerics/convection/SF_TENO5.cpp:44-92.
+ * All-cutoff is an error there (process exit); here it raises a bit. */
+double sfo_teno5_core(const double v[5], unsigned *err)
+{
+    for (int s = 0; s < 5; ++s)
+        if (!isfinite(v[s])) { *err |= SFO_ERR_WENO; return NAN; }
+    const double p0 = (2.0 * v[0] - 7.0 * v[1] + 11.0 * v[2]) / 6.0;
+    const double p1 = (-1.0 * v[1] + 5.0 * v[2] + 2.0 * v[3]) / 6.0;
+    const double p2 = (2.0 * v[2] + 5.0 * v[3] - 1.0 * v[4]) / 6.0;
+    const double b0 = 13.0 / 12.0 * sq(v[0] - 2.0 * v[1] + v[2])
+                    + 0.25 * sq(v[0] - 4.0 * v[1] + 3.0 * v[2]);
+    const double b1 = 13.0 / 12.0 * sq(v[1] - 2.0 * v[2] + v[3])
+                    + 0.25 * sq(v[1] - v[3]);
+    const double b2 = 13.0 / 12.0 * sq(v[2] - 2.0 * v[3] + v[4])
+                    + 0.25 * sq(3.0 * v[2] - 4.0 * v[3] + v[4]);
+    const double d0 = 0.1, d1 = 0.6, d2 = 0.3;
+    const double eps = 1.0e-40, ct = 1.0e-5;
+    const double tau5 = fabs(b0 - b2);
+    const double g0 = pow(1.0 + tau5 / (b0 + eps), 6.0);
+    const double g1 = pow(1.0 + tau5 / (b1 + eps), 6.0);
+    const double g2 = pow(1.0 + tau5 / (b2 + eps), 6.0);
+    const double gs = g0 + g1 + g2;
+    if (!isfinite(gs) || gs <= 0.0) { *err |= SFO_ERR_WENO; return NAN; }
+    const double c0 = (g0 / gs < ct) ? 0.0 : 1.0;
+    const double c1 = (g1 / gs < ct) ? 0.0 : 1.0;
+    const double c2 = (g2 / gs < ct) ? 0.0 : 1.0;
+    const double active = d0 * c0 + d1 * c1 + d2 * c2;
+    if (active <= 0.0) { *err |= SFO_ERR_TENO; return NAN; }
+    return (d0 * c0 * p0 + d1 * c1 * p1 + d2 * c2 * p2) / active;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Thermodynamic closures                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* Legacy pressure with positivity check, the form used by the
+ * characteristic matrix.  Numerics::pressure + requirePhysicalState,
+ * src/methods/numerics/SF_utility.h:24-29,50-66. */
+static int legacy_pressure(const double q[5], double gamma, double *p)
+{
+    int bad = !isfinite(q[0]) || !isfinite(q[1]) || !isfinite(q[2])
+           || !isfinite(q[3]) || !isfinite(q[4]) || q[0] <= 0.0;
+    if (bad) return -1;
+    double invRho = 1.0 / q[0];
+    double ke = 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * invRho;
+    double pr = (gamma - 1.0) * (q[4] - ke);
+    if (!isfinite(pr) || pr <= 0.0) return -1;
+    *p = pr;
+    return 0;
+}
+
+typedef struct { double rho, u[3], p, c; } closed_t;
+
+/* SingleFluidStateModel::close + PerfectGasEOS::fromDensityEnergy,
+ * src/models/physics/fluidStateModel/SF_singleFluidStateModel.cpp:49-71,
+ * src/models/physics/EOS/SF_perfectGasEOS.h:24-32 (reference energy 0).
+ * Temperature positivity is equivalent to pressure positivity for rho>0. */
+static int close_state(const double q[5], double gamma, closed_t *s)
+{
+    if (!isfinite(q[0]) || q[0] <= 0.0) return -1;
+    for (int d = 1; d < 4; ++d) if (!isfinite(q[d])) return -1;
+    const double rho = q[0];
+    s->rho = rho;
+    for (int d = 0; d < 3; ++d) s->u[d] = q[1 + d] / rho;
+    const double kinetic = 0.5 * rho
+        * (s->u[0] * s->u[0] + s->u[1] * s->u[1] + s->u[2] * s->u[2]);
+    const double internal = q[4] - kinetic;
+    const double e = internal / rho;
+    const double thermal = e - 0.0;
+    const double p = (gamma - 1.0) * rho * thermal;
+    const double c = sqrt(gamma * p / rho);
+    if (!isfinite(e) || !isfinite(p) || p <= 0.0 || !isfinite(c) || c <= 0.0)
+        return -1;
+    s->p = p;
+    s->c = c;
+    return 0;
+}
+
+/* SingleFluidStateModel::totalEnergyFromPressure ->
+ * PerfectGasEOS::fromDensityPressure -> totalEnergyFromThermo,
+ * SF_singleFluidStateModel.cpp:73-109, SF_perfectGasEOS.h:44-56. */
+static double energy_from_pressure(const double q[5], double gamma, double p)
+{
+    const double thermal = p / ((gamma - 1.0) * q[0]);
+    const double e = 0.0 + thermal;
+    const double inverseDensity = 1.0 / q[0];
+    const double kinetic = 0.5 * inverseDensity
+        * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    return q[0] * e + kinetic;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Characteristic decomposition                                            */
+/* ---------------------------------------------------------------------- */
+
+/* Riemann::buildCharacteristicMatrix,
+ * src/methods/numerics/convection/SF_riemann.cpp:69-123.  The tangent
+ * basis (l, m) is chosen exactly as there (F5 in the survey). */
+int sfo_char_matrix(const double Q[5], const double n[3], double gamma,
+                    double L[25], double R[25])
+{
+    double p;
+    if (legacy_pressure(Q, gamma, &p)) return -1;
+    double r = Q[0];
+    double invR = 1.0 / r;
+    double u = Q[1] * invR, v = Q[2] * invR, w = Q[3] * invR;
+    double e = Q[4];
+    double q2 = u * u + v * v + w * w;
+    double c = sqrt(gamma * p * invR);
+    double H = (e + p) * invR;
+    double n1 = n[0], n2 = n[1], n3 = n[2];
+    double l1, l2, l3, m1, m2, m3;
+    if (fabs(n3) <= fabs(n2)) {
+        double ss = sqrt(n1 * n1 + n2 * n2);
+        l1 = -n2 / ss; l2 = n1 / ss; l3 = 0.0;
+    } else {
+        double ss = sqrt(n1 * n1 + n3 * n3);
+        l1 = -n3 / ss; l2 = 0.0; l3 = n1 / ss;
+    }
+    m1 = n2 * l3 - n3 * l2;
+    m2 = n3 * l1 - n1 * l3;
+    m3 = n1 * l2 - n2 * l1;
+    double un = u * n1 + v * n2 + w * n3;
+    double ul = u * l1 + v * l2 + w * l3;
+    double um = u * m1 + v * m2 + w * m3;
+    double b1 = (gamma - 1.0) / (c * c);
+    double b2 = 0.5 * b1;
+    double x1 = 1.0 / (2.0 * c);
+    double Lm[25] = {
+        1.0 - b2 * q2, b1 * u, b1 * v, b1 * w, -b1,
+        -ul, l1, l2, l3, 0.0,
+        -um, m1, m2, m3, 0.0,
+        0.5 * b2 * q2 + x1 * un, -x1 * n1 - b2 * u, -x1 * n2 - b2 * v,
+        -x1 * n3 - b2 * w, b2,
+        0.5 * b2 * q2 - x1 * un, x1 * n1 - b2 * u, x1 * n2 - b2 * v,
+        x1 * n3 - b2 * w, b2};
+    double Rm[25] = {
+        1.0, 0.0, 0.0, 1.0, 1.0,
+        u, l1, m1, u - c * n1, u + c * n1,
+        v, l2, m2, v - c * n2, v + c * n2,
+        w, l3, m3, w - c * n3, w + c * n3,
+        0.5 * q2, ul, um, H - c * un, H + c * un};
+    memcpy(L, Lm, sizeof Lm);
+    memcpy(R, Rm, sizeof Rm);
+    return 0;
+}
+
+/* Riemann::gemv5, src/methods/numerics/convection/SF_riemann.h:33-37. */
+void sfo_gemv5(const double M[25], const double V[5], double out[5])
+{
+    for (int i = 0; i < 5; ++i)
+        out[i] = M[i * 5 + 0] * V[0] + M[i * 5 + 1] * V[1]
+               + M[i * 5 + 2] * V[2] + M[i * 5 + 3] * V[3]
+               + M[i * 5 + 4] * V[4];
+}
+
+/* ---------------------------------------------------------------------- */
+/* Face fluxes                                                             */
+/* ---------------------------------------------------------------------- */
+
+/* physicalFlux, src/methods/numerics/Flux/SF_rusanovEOS.cpp:14-32. */
+static void physical_flux(const double q[5], const closed_t *s,
+                          const double n[3], double f[5])
+{
+    const double un = s->u[0] * n[0] + s->u[1] * n[1] + s->u[2] * n[2];
+    f[0] = q[0] * un;
+    for (int d = 0; d < 3; ++d) f[1 + d] = q[1 + d] * un + s->p * n[d];
+    f[4] = (q[4] + s->p) * un;
+}
+
+int sfo_physical_flux(const double q[5], const double n[3], double gamma,
+                      double out[5])
+{
+    closed_t s;
+    if (close_state(q, gamma, &s)) return -1;
+    physical_flux(q, &s, n, out);
+    return 0;
+}
+
+/* RusanovEOS::faceFlux, src/methods/numerics/Flux/SF_rusanovEOS.cpp:47-77. */
+int sfo_rusanov(const double qL[5], const double qR[5], const double n[3],
+                double gamma, double out[5])
+{
+    closed_t l, r;
+    if (close_state(qL, gamma, &l) || close_state(qR, gamma, &r)) return -1;
+    double fL[5], fR[5];
+    physical_flux(qL, &l, n, fL);
+    physical_flux(qR, &r, n, fR);
+    const double unL = l.u[0] * n[0] + l.u[1] * n[1] + l.u[2] * n[2];
+    const double unR = r.u[0] * n[0] + r.u[1] * n[1] + r.u[2] * n[2];
+    const double a = fabs(unL) + l.c, b = fabs(unR) + r.c;
+    const double lambda = a > b ? a : b;   /* std::max(a, b) */
+    if (!isfinite(lambda) || lambda <= 0.0) return -1;
+    for (int v = 0; v < 5; ++v)
+        out[v] = 0.5 * (fL[v] + fR[v]) - 0.5 * lambda * (qR[v] - qL[v]);
+    return 0;
+}
+
+/* HLLC (no reference counterpart: the flux enum of the reference has no
+ * HLLC member).  Toro, 3rd ed., section 10.4-10.6:
+ *  - wave speeds: Einfeldt (1988) estimates from Roe averages,
+ *      S_L = min(u_nL - c_L, u~_n - c~),  S_R = max(u_nR + c_R, u~_n + c~);
+ *  - contact speed S* from Toro eq. (10.37);
+ *  - star states from Toro eq. (10.39), written for a unit normal n so the
+ *    tangential velocity of each side is carried unchanged;
+ *  - flux from Toro eq. (10.38) (the four-region choice by signs of
+ *    S_L, S*, S_R). */
+int sfo_hllc(const double qL[5], const double qR[5], const double n[3],
+             double gamma, double out[5])
+{
+    closed_t l, r;
+    if (close_state(qL, gamma, &l) || close_state(qR, gamma, &r)) return -1;
+    const double unL = l.u[0] * n[0] + l.u[1] * n[1] + l.u[2] * n[2];
+    const double unR = r.u[0] * n[0] + r.u[1] * n[1] + r.u[2] * n[2];
+    const double sL = sqrt(l.rho), sR = sqrt(r.rho);
+    const double inv = 1.0 / (sL + sR);
+    double ut[3];
+    for (int d = 0; d < 3; ++d) ut[d] = (sL * l.u[d] + sR * r.u[d]) * inv;
+    const double HL = (qL[4] + l.p) / l.rho, HR = (qR[4] + r.p) / r.rho;
+    const double Ht = (sL * HL + sR * HR) * inv;
+    const double ct2 = (gamma - 1.0)
+        * (Ht - 0.5 * (ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]));
+    if (!(ct2 > 0.0) || !isfinite(ct2)) return -1;
+    const double ct = sqrt(ct2);
+    const double unt = ut[0] * n[0] + ut[1] * n[1] + ut[2] * n[2];
+    const double SL = fmin(unL - l.c, unt - ct);
+    const double SR = fmax(unR + r.c, unt + ct);
+    const double mL = l.rho * (SL - unL), mR = r.rho * (SR - unR);
+    const double den = mL - mR;
+    if (!(den != 0.0)) return -1;
+    const double Ss = (r.p - l.p + unL * mL - unR * mR) / den;
+    if (!isfinite(SL) || !isfinite(SR) || !isfinite(Ss)) return -1;
+    double f[5];
+    if (0.0 <= SL) {
+        physical_flux(qL, &l, n, out);
+        return 0;
+    }
+    if (SR <= 0.0) {
+        physical_flux(qR, &r, n, out);
+        return 0;
+    }
+    const int left = 0.0 <= Ss;
+    const double *q = left ? qL : qR;
+    const closed_t *s = left ? &l : &r;
+    const double un = left ? unL : unR;
+    const double SK = left ? SL : SR;
+    const double mK = left ? mL : mR;
+    physical_flux(q, s, n, f);
+    const double scale = mK / (SK - Ss);
+    double us[5];
+    us[0] = scale;
+    for (int d = 0; d < 3; ++d) us[1 + d] = scale * (s->u[d] + (Ss - un) * n[d]);
+    us[4] = scale * (q[4] / s->rho + (Ss - un) * (Ss + s->p / mK));
+    for (int v = 0; v < 5; ++v) out[v] = f[v] + SK * (us[v] - q[v]);
+    return 0;
+}
+
+/* Flux::characteristicWENOFaceFlux<6> with the scalar-stencil mapping of
+ * fillScalarWENOStencils<6>,
+ * src/solver/discretization/convection/SF_WENO.h:79-98,185-245. */
+int sfo_char_face_flux(const double st[30], const double cof[3], int scheme,
+                       int flux, double gamma, double out[5])
+{
+    const double smag = sqrt(cof[0] * cof[0] + cof[1] * cof[1] + cof[2] * cof[2]);
+    if (!isfinite(smag) || smag <= 1.0e-300) return SFO_ERR_ARG;
+    const double n[3] = {cof[0] / smag, cof[1] / smag, cof[2] / smag};
+    const double area = smag;
+    double qa[5];
+    for (int v = 0; v < 5; ++v) qa[v] = 0.5 * (st[2 * 5 + v] + st[3 * 5 + v]);
+    double L[25], R[25];
+    if (sfo_char_matrix(qa, n, gamma, L, R)) return SFO_ERR_CHAR;
+    double ch[6][5];
+    for (int s = 0; s < 6; ++s) sfo_gemv5(L, st + 5 * s, ch[s]);
+    double wL[5], wR[5];
+    unsigned err = 0;
+    for (int v = 0; v < 5; ++v) {
+        double left[5], right[5];
+        for (int s = 0; s < 5; ++s) {
+            left[s] = ch[s][v];
+            right[s] = ch[5 - s][v];
+        }
+        if (scheme == SFO_TENO5) {
+            wL[v] = sfo_teno5_core(left, &err);
+            wR[v] = sfo_teno5_core(right, &err);
+        } else {
+            wL[v] = sfo_weno5_core(left, &err);
+            wR[v] = sfo_weno5_core(right, &err);
+        }
+        if (err) return (int)err;
+        if (!isfinite(wL[v]) || !isfinite(wR[v])) return SFO_ERR_WENO;
+    }
+    double qL[5], qR[5], f[5];
+    sfo_gemv5(R, wL, qL);
+    sfo_gemv5(R, wR, qR);
+    int rc = flux == SFO_HLLC ? sfo_hllc(qL, qR, n, gamma, f)
+                              : sfo_rusanov(qL, qR, n, gamma, f);
+    if (rc) return SFO_ERR_FLUX;
+    for (int v = 0; v < 5; ++v) out[v] = f[v] * area;
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Grid                                                                    */
+/* ---------------------------------------------------------------------- */
+
+struct sfo_grid {
+    sfo_desc desc;
+    int g[3], m[3];
+    size_t cells;
+    double *q, *q0, *k[4], *fx[3];
+    double jac, metric[3], cof[3];
+    unsigned err;
+};
+
+static size_t at(const sfo_grid *G, int i, int j, int k)
+{
+    return ((size_t)k * G->m[1] + j) * G->m[0] + i;
+}
+
+/* A node is "solved" (Math::forFluidInterior,
+ * src/methods/numerics/structured/SF_iteration.h:37-58) unless it lies on
+ * the boundary plane of a transmissive axis, which the boundary
+ * applicator marks as a solver-boundary point (SF_applicator.h:162-187). */
+static int solved(const sfo_grid *G, const int p[3])
+{
+    for (int d = 0; d < 3; ++d) {
+        if (!G->desc.active[d] || G->desc.bc[d] != SFO_TRANSMISSIVE) continue;
+        if (p[d] == G->g[d] || p[d] == G->g[d] + G->desc.n[d] - 1) return 0;
+    }
+    return 1;
+}
+
+sfo_grid *sfo_grid_create(const sfo_desc *desc)
+{
+    for (int d = 0; d < 3; ++d) {
+        if (desc->n[d] < 1) return NULL;
+        if (desc->active[d] && desc->n[d] < 4) return NULL;
+        if (!desc->active[d] && desc->n[d] != 1) return NULL;
+        if (!(desc->d[d] > 0.0)) return NULL;
+    }
+    sfo_grid *G = (sfo_grid *)calloc(1, sizeof *G);
+    if (!G) return NULL;
+    G->desc = *desc;
+    G->cells = 1;
+    for (int d = 0; d < 3; ++d) {
+        G->g[d] = desc->active[d] ? 3 : 0;
+        G->m[d] = desc->n[d] + 2 * G->g[d];
+        G->cells *= (size_t)G->m[d];
+    }
+    size_t bytes = G->cells * 5 * sizeof(double);
+    G->q = (double *)calloc(1, bytes);
+    G->q0 = (double *)calloc(1, bytes);
+    for (int s = 0; s < 4; ++s) G->k[s] = (double *)calloc(1, bytes);
+    for (int d = 0; d < 3; ++d) G->fx[d] = (double *)calloc(1, bytes);
+    /* uniform Cartesian metrics: xi_x = 1/dx, ..., Jac = 1/J = 1/(dx dy dz)
+     * (Mesh::computeMetrics, src/infrastructure/mesh/SF_mesh.cpp:320-338),
+     * face cofactor = average of metric/Jac at both nodes
+     * (CanonicalFace::geometry, SF_canonicalFace.h:87-97). */
+    G->jac = 1.0 / (desc->d[0] * desc->d[1] * desc->d[2]);
+    for (int d = 0; d < 3; ++d) {
+        G->metric[d] = 1.0 / desc->d[d];
+        double c = G->metric[d] / G->jac;
+        G->cof[d] = 0.5 * (c + c);
+    }
+    return G;
+}
+
+void sfo_grid_destroy(sfo_grid *G)
+{
+    if (!G) return;
+    free(G->q); free(G->q0);
+    for (int s = 0; s < 4; ++s) free(G->k[s]);
+    for (int d = 0; d < 3; ++d) free(G->fx[d]);
+    free(G);
+}
+
+unsigned sfo_error(const sfo_grid *G) { return G->err; }
+
+#define FOR_NODES(G, p)                                                    \
+    for (p[2] = (G)->g[2]; p[2] < (G)->g[2] + (G)->desc.n[2]; ++p[2])      \
+        for (p[1] = (G)->g[1]; p[1] < (G)->g[1] + (G)->desc.n[1]; ++p[1])  \
+            for (p[0] = (G)->g[0]; p[0] < (G)->g[0] + (G)->desc.n[0]; ++p[0])
+
+int sfo_set_state(sfo_grid *G, const double *soa)
+{
+    const size_t N = (size_t)G->desc.n[0] * G->desc.n[1] * G->desc.n[2];
+    int p[3];
+    FOR_NODES(G, p) {
+        size_t u = ((size_t)(p[2] - G->g[2]) * G->desc.n[1] + (p[1] - G->g[1]))
+                 * G->desc.n[0] + (p[0] - G->g[0]);
+        double *q = G->q + 5 * at(G, p[0], p[1], p[2]);
+        for (int v = 0; v < 5; ++v) q[v] = soa[v * N + u];
+    }
+    return 0;
+}
+
+int sfo_get_state(const sfo_grid *G, double *soa)
+{
+    const size_t N = (size_t)G->desc.n[0] * G->desc.n[1] * G->desc.n[2];
+    int p[3];
+    FOR_NODES(G, p) {
+        size_t u = ((size_t)(p[2] - G->g[2]) * G->desc.n[1] + (p[1] - G->g[1]))
+                 * G->desc.n[0] + (p[0] - G->g[0]);
+        const double *q = G->q + 5 * at(G, p[0], p[1], p[2]);
+        for (int v = 0; v < 5; ++v) soa[v * N + u] = q[v];
+    }
+    return 0;
+}
+
+/* Iterate the two boundary planes of axis d over the physical range of the
+ * other axes; body gets the boundary node, its inner neighbour and the
+ * direction (+1 upward into ghosts on the high side, -1 on the low side). */
+typedef void (*plane_fn)(sfo_grid *, const int b[3], const int src[3], int d,
+                         int dir);
+
+static void for_planes(sfo_grid *G, int d, plane_fn fn)
+{
+    int e1 = (d + 1) % 3, e2 = (d + 2) % 3;
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) { lo[a] = G->g[a]; hi[a] = G->g[a] + G->desc.n[a]; }
+    for (int side = 0; side < 2; ++side) {
+        int p[3];
+        for (p[e2] = lo[e2]; p[e2] < hi[e2]; ++p[e2])
+            for (p[e1] = lo[e1]; p[e1] < hi[e1]; ++p[e1]) {
+                int b[3] = {p[0], p[1], p[2]}, s[3];
+                b[d] = side == 0 ? G->g[d] : G->g[d] + G->desc.n[d] - 1;
+                memcpy(s, b, sizeof s);
+                s[d] += side == 0 ? 1 : -1;
+                fn(G, b, s, d, side == 0 ? -1 : 1);
+            }
+    }
+}
+
+/* zeroGradient for density and momentum: the boundary node takes its
+ * inner neighbour's value and the ghosts take the boundary node's value
+ * (Algebraic::ZeroGradient::applyScalar/applyVector3,
+ * src/solver/boundary/reconstruction/linear/SF_zeroGradient.cpp:88-130). */
+static void zg_copy4(sfo_grid *G, const int b[3], const int s[3], int d, int dir)
+{
+    double *qb = G->q + 5 * at(G, b[0], b[1], b[2]);
+    const double *qs = G->q + 5 * at(G, s[0], s[1], s[2]);
+    for (int v = 0; v < 4; ++v) qb[v] = qs[v];
+    for (int l = 1; l <= G->g[d]; ++l) {
+        int gp[3] = {b[0], b[1], b[2]};
+        gp[d] += dir * l;
+        double *qg = G->q + 5 * at(G, gp[0], gp[1], gp[2]);
+        for (int v = 0; v < 4; ++v) qg[v] = qb[v];
+    }
+}
+
+static double pressure_at(const sfo_grid *G, const double *q)
+{
+    closed_t s;
+    if (close_state(q, G->desc.gamma, &s)) return NAN;
+    return s.p;
+}
+
+/* zeroGradient for pressure, converted to total energy:
+ * updateEnergyFromPressure ZERO_GRADIENT branch,
+ * src/solver/boundary/SF_thermodynamicClosure.cpp:219-281. */
+static void zg_energy(sfo_grid *G, const int b[3], const int s[3], int d, int dir)
+{
+    const double gamma = G->desc.gamma;
+    double *qb = G->q + 5 * at(G, b[0], b[1], b[2]);
+    const double *qs = G->q + 5 * at(G, s[0], s[1], s[2]);
+    double ps = pressure_at(G, qs);
+    if (!isfinite(ps) || ps <= 0.0) { G->err |= SFO_ERR_STATE; return; }
+    qb[4] = energy_from_pressure(qb, gamma, ps);
+    double pb = pressure_at(G, qb);
+    if (!isfinite(pb) || pb <= 0.0) { G->err |= SFO_ERR_STATE; return; }
+    for (int l = 1; l <= G->g[d]; ++l) {
+        int gp[3] = {b[0], b[1], b[2]};
+        gp[d] += dir * l;
+        double *qg = G->q + 5 * at(G, gp[0], gp[1], gp[2]);
+        qg[4] = energy_from_pressure(qg, gamma, pb);
+    }
+}
+
+/* Periodic fill (no reference counterpart): ghost layer l on either side
+ * of axis d copies the unique node at (index mod n). */
+static void periodic_fill(sfo_grid *G, int d)
+{
+    int e1 = (d + 1) % 3, e2 = (d + 2) % 3;
+    const int n = G->desc.n[d], g = G->g[d];
+    int p[3];
+    for (p[e2] = G->g[e2]; p[e2] < G->g[e2] + G->desc.n[e2]; ++p[e2])
+        for (p[e1] = G->g[e1]; p[e1] < G->g[e1] + G->desc.n[e1]; ++p[e1])
+            for (int l = 0; l < g; ++l) {
+                int lo[3] = {p[0], p[1], p[2]}, hi[3] = {p[0], p[1], p[2]};
+                int slo[3] = {p[0], p[1], p[2]}, shi[3] = {p[0], p[1], p[2]};
+                lo[d] = l;                 slo[d] = g + ((l - g) % n + n) % n;
+                hi[d] = g + n + l;         shi[d] = g + l % n;
+                memcpy(G->q + 5 * at(G, lo[0], lo[1], lo[2]),
+                       G->q + 5 * at(G, slo[0], slo[1], slo[2]), 5 * sizeof(double));
+                memcpy(G->q + 5 * at(G, hi[0], hi[1], hi[2]),
+                       G->q + 5 * at(G, shi[0], shi[1], shi[2]), 5 * sizeof(double));
+            }
+}
+
+/* Boundary::Applicator::apply order (SF_applicator.h:51-83): density and
+ * momentum for every set, then energy from pressure for every set; sets
+ * are visited axis by axis (x, y, z).  Periodic axes are filled last. */
+int sfo_apply_bc(sfo_grid *G)
+{
+    for (int d = 0; d < 3; ++d)
+        if (G->desc.active[d] && G->desc.bc[d] == SFO_TRANSMISSIVE)
+            for_planes(G, d, zg_copy4);
+    for (int d = 0; d < 3; ++d)
+        if (G->desc.active[d] && G->desc.bc[d] == SFO_TRANSMISSIVE)
+            for_planes(G, d, zg_energy);
+    for (int d = 0; d < 3; ++d)
+        if (G->desc.active[d] && G->desc.bc[d] == SFO_PERIODIC)
+            periodic_fill(G, d);
+    return G->err ? -1 : 0;
+}
+
+/* Flux::computeDirectionalFlux (SF_WENO.h:257-364) over Math::forFaces
+ * (SF_iteration.h:61-82): faces i+1/2 for i = g-1 .. g+n-1 on the axis,
+ * physical range on the others; the lower node owns the face
+ * (Math::storeFlux, SF_faceAssembly.h:26-40). */
+static int compute_fluxes(sfo_grid *G, int d)
+{
+    int p[3], lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) { lo[a] = G->g[a]; hi[a] = G->g[a] + G->desc.n[a]; }
+    lo[d] = G->g[d] - 1;
+    double cof[3] = {0.0, 0.0, 0.0};
+    cof[d] = G->cof[d];
+    for (p[2] = lo[2]; p[2] < hi[2]; ++p[2])
+        for (p[1] = lo[1]; p[1] < hi[1]; ++p[1])
+            for (p[0] = lo[0]; p[0] < hi[0]; ++p[0]) {
+                double st[30];
+                for (int s = 0; s < 6; ++s) {
+                    int r[3] = {p[0], p[1], p[2]};
+                    r[d] += s - 2;
+                    memcpy(st + 5 * s, G->q + 5 * at(G, r[0], r[1], r[2]),
+                           5 * sizeof(double));
+                }
+                double f[5];
+                int rc = sfo_char_face_flux(st, cof, G->desc.scheme,
+                                            G->desc.flux, G->desc.gamma, f);
+                if (rc) { G->err |= (unsigned)rc; return -1; }
+                for (int v = 0; v < 5; ++v) {
+                    if (!isfinite(f[v])) { G->err |= SFO_ERR_FLUX; return -1; }
+                    G->fx[d][5 * at(G, p[0], p[1], p[2]) + v] = f[v];
+                }
+            }
+    return 0;
+}
+
+/* Math::localSpatialResidual (SF_residualAssembly.h:12-32) on solved
+ * nodes, stored like Time::Explicit::snapshotRHS (SF_explicit.cpp:181-191):
+ * R = Jac * sum_active (F_d(i) - F_d(i-1)) - source (source = 0 here). */
+static int assemble_rhs(sfo_grid *G, double *out)
+{
+    for (int d = 0; d < 3; ++d)
+        if (G->desc.active[d] && compute_fluxes(G, d)) return -1;
+    memset(out, 0, G->cells * 5 * sizeof(double));
+    int p[3];
+    FOR_NODES(G, p) {
+        if (!solved(G, p)) continue;
+        size_t c = at(G, p[0], p[1], p[2]);
+        for (int v = 0; v < 5; ++v) {
+            double div = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                if (!G->desc.active[d]) continue;
+                int r[3] = {p[0], p[1], p[2]};
+                r[d] -= 1;
+                div += G->fx[d][5 * c + v] - G->fx[d][5 * at(G, r[0], r[1], r[2]) + v];
+            }
+            out[5 * c + v] = G->jac * div - 0.0;
+        }
+    }
+    return 0;
+}
+
+int sfo_residual(sfo_grid *G, double *soa)
+{
+    if (sfo_apply_bc(G)) return -1;
+    if (assemble_rhs(G, G->k[0])) return -1;
+    const size_t N = (size_t)G->desc.n[0] * G->desc.n[1] * G->desc.n[2];
+    int p[3];
+    FOR_NODES(G, p) {
+        size_t u = ((size_t)(p[2] - G->g[2]) * G->desc.n[1] + (p[1] - G->g[1]))
+                 * G->desc.n[0] + (p[0] - G->g[0]);
+        const double *r = G->k[0] + 5 * at(G, p[0], p[1], p[2]);
+        for (int v = 0; v < 5; ++v) soa[v * N + u] = r[v];
+    }
+    return 0;
+}
+
+/* RusanovEOS::deltaT (SF_rusanovEOS.cpp:110-147) over solved nodes. */
+int sfo_compute_dt(sfo_grid *G, double cfl, double *dt)
+{
+    double result = DBL_MAX;
+    int p[3];
+    FOR_NODES(G, p) {
+        if (!solved(G, p)) continue;
+        closed_t s;
+        if (close_state(G->q + 5 * at(G, p[0], p[1], p[2]), G->desc.gamma, &s)) {
+            G->err |= SFO_ERR_DT;
+            return -1;
+        }
+        double spectral = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            if (!G->desc.active[d]) continue;
+            double m[3] = {0.0, 0.0, 0.0};
+            m[d] = G->metric[d];
+            double un = fabs(s.u[0] * m[0] + s.u[1] * m[1] + s.u[2] * m[2]);
+            spectral += un + s.c * sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        }
+        if (!isfinite(spectral) || spectral <= 0.0) { G->err |= SFO_ERR_DT; return -1; }
+        double local = cfl / spectral;
+        if (local < result) result = local;
+    }
+    if (!isfinite(result) || result == DBL_MAX) { G->err |= SFO_ERR_DT; return -1; }
+    *dt = result;
+    return 0;
+}
+
+/* validateEquationState (SF_equationState.h:25-57) over solved nodes. */
+static int validate(sfo_grid *G)
+{
+    int p[3];
+    FOR_NODES(G, p) {
+        if (!solved(G, p)) continue;
+        closed_t s;
+        if (close_state(G->q + 5 * at(G, p[0], p[1], p[2]), G->desc.gamma, &s)) {
+            G->err |= SFO_ERR_STATE;
+            return -1;
+        }
+    }
+    return 0;
+}
+
+/* One physical step: Time::Explicit::begin/executeStage
+ * (SF_explicit.cpp:193-238,287-530) with the tableaux of
+ * SF_builtinProviders.cpp:85-127; the boundary state is prepared before
+ * every right-hand side and once more at commit
+ * (SF_conservativeRHS.cpp:137-138, SF_flowOperations.cpp:648-652). */
+int sfo_step(sfo_grid *G, double dt)
+{
+    int p[3];
+    const int scheme = G->desc.time;
+    if (scheme == SFO_EULER) {
+        if (sfo_apply_bc(G) || assemble_rhs(G, G->k[0])) return -1;
+        FOR_NODES(G, p) {
+            if (!solved(G, p)) continue;
+            size_t c = 5 * at(G, p[0], p[1], p[2]);
+            for (int v = 0; v < 5; ++v) G->q[c + v] -= dt * G->k[0][c + v];
+            double pr;
+            if (legacy_pressure(G->q + c, G->desc.gamma, &pr)) {
+                G->err |= SFO_ERR_STATE;
+                return -1;
+            }
+        }
+        if (validate(G)) return -1;
+        return sfo_apply_bc(G);
+    }
+    /* begin: snapshot q0 on solved nodes */
+    memset(G->q0, 0, G->cells * 5 * sizeof(double));
+    FOR_NODES(G, p) {
+        if (!solved(G, p)) continue;
+        size_t c = 5 * at(G, p[0], p[1], p[2]);
+        memcpy(G->q0 + c, G->q + c, 5 * sizeof(double));
+    }
+    if (scheme == SFO_SSPRK3) {
+        static const double base[3] = {0.0, 0.75, 1.0 / 3.0};
+        static const double incr[3] = {1.0, 0.25, 2.0 / 3.0};
+        for (int st = 0; st < 3; ++st) {
+            if (sfo_apply_bc(G) || assemble_rhs(G, G->k[0])) return -1;
+            FOR_NODES(G, p) {
+                if (!solved(G, p)) continue;
+                size_t c = 5 * at(G, p[0], p[1], p[2]);
+                for (int v = 0; v < 5; ++v) {
+                    const double euler = G->q[c + v] - dt * G->k[0][c + v];
+                    G->q[c + v] = base[st] * G->q0[c + v] + incr[st] * euler;
+                }
+            }
+            if (validate(G)) return -1;
+        }
+        return sfo_apply_bc(G);
+    }
+    if (scheme == SFO_RK4) {
+        static const double incr[3] = {0.5, 0.5, 1.0};
+        static const double w[4] = {1.0, 2.0, 2.0, 1.0};
+        const double divisor = 6.0;
+        for (int st = 0; st < 4; ++st) {
+            if (sfo_apply_bc(G) || assemble_rhs(G, G->k[st])) return -1;
+            if (validate(G)) return -1;
+            FOR_NODES(G, p) {
+                if (!solved(G, p)) continue;
+                size_t c = 5 * at(G, p[0], p[1], p[2]);
+                for (int v = 0; v < 5; ++v) {
+                    if (st < 3) {
+                        G->q[c + v] = G->q0[c + v] - incr[st] * dt * G->k[st][c + v];
+                    } else {
+                        G->q[c + v] = G->q0[c + v]
+                            - dt / divisor * (w[0] * G->k[0][c + v]
+                                              + w[1] * G->k[1][c + v]
+                                              + w[2] * G->k[2][c + v]
+                                              + w[3] * G->k[3][c + v]);
+                    }
+                }
+            }
+            if (validate(G)) return -1;
+        }
+        return sfo_apply_bc(G);
+    }
+    G->err |= SFO_ERR_ARG;
+    return -1;
+}
+
+/* flow.step.prepare -> flow.dt.compute (min with maxDeltaT) -> step
+ * (SF_flowOperations.cpp:471-497). */
+int sfo_run(sfo_grid *G, int nsteps, double cfl, double max_dt, double *dt_seq)
+{
+    for (int s = 0; s < nsteps; ++s) {
+        double dt;
+        if (sfo_apply_bc(G) || sfo_compute_dt(G, cfl, &dt)) return -1;
+        if (max_dt < dt) dt = max_dt;
+        if (dt_seq) dt_seq[s] = dt;
+        if (sfo_step(G, dt)) return -1;
+    }
+    return 0;
+}
