@@ -1,0 +1,162 @@
+/*
+ * sonic_amd.h -- C ABI of the MI355X (gfx950) implementation of
+ * sonicSolver's density-based single-fluid explicit hot path:
+ * characteristic WENO5-JS / TENO5 face reconstruction, Rusanov / HLLC face
+ * fluxes, the ConservativeResidual flux divergence, the Euler / SSPRK3 /
+ * RK4 stage updates, on-device ghost fill and the CFL time-step reduction.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative SF_E_* code on
+ *    failure; sf_last_error() returns the message of the last failure on
+ *    the calling thread.
+ *  - State crosses the boundary as plain FP64 arrays, variable-major over
+ *    the unique nodes: soa[v*N + (k*ny + j)*nx + i], N = nx*ny*nz,
+ *    v = {rho, rhoU, rhoV, rhoW, rhoE}.  Host arrays are owned by the
+ *    caller; device memory is owned by the library.
+ *  - A handle is used from one host thread at a time; all its work is
+ *    queued on one HIP stream owned by the handle.
+ *
+ * Citations name the reference interface each entry point stands in for
+ * (paths relative to the sonicSolver source tree).
+ */
+#ifndef SONIC_AMD_H
+#define SONIC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enumerations ------------------------------------------------------ */
+/* FDM::ConvectionScheme, core/config/types/SF_termRecipe.h:14 */
+enum { SF_SCHEME_WENO5 = 0, SF_SCHEME_TENO5 = 1 };
+/* FDM::FluxSplitter, SF_termRecipe.h:34-41; HLLC has no reference member */
+enum { SF_FLUX_RUSANOV = 0, SF_FLUX_HLLC = 1 };
+/* FDM::TimeRecipeId, core/config/types/SF_timeRecipe.h:10 */
+enum { SF_TIME_EULER = 0, SF_TIME_SSPRK3 = 1, SF_TIME_RK4 = 2 };
+/* zeroGradient (transmissive) boundary sets, or periodic (no reference
+ * counterpart: BCType has no periodic kind) */
+enum { SF_BC_TRANSMISSIVE = 0, SF_BC_PERIODIC = 1 };
+
+/* error codes */
+enum {
+    SF_OK = 0,
+    SF_E_ARG = -1,        /* invalid argument                               */
+    SF_E_HIP = -2,        /* HIP runtime failure                            */
+    SF_E_NUMERIC = -3,    /* device raised a numerical fault (see flags)    */
+    SF_E_MEMORY = -4,     /* device allocation failed                       */
+    SF_E_STATE = -5       /* call out of order                              */
+};
+
+/* device error flags (the reference terminates the process on each) */
+enum {
+    SF_FLAG_STATE = 1u,   /* non-physical state after a stage:
+                             validateEquationState, SF_equationState.h:25-57 */
+    SF_FLAG_CHAR = 2u,    /* non-physical face-average state:
+                             buildCharacteristicMatrix, SF_riemann.cpp:72-74 */
+    SF_FLAG_TENO = 4u,    /* TENO5 rejected all substencils:
+                             SF_TENO5.cpp:79-89                              */
+    SF_FLAG_FLUX = 8u,    /* non-finite face flux: storeFlux,
+                             SF_faceAssembly.h:26-40                         */
+    SF_FLAG_DT = 16u      /* invalid spectral radius: RusanovEOS::deltaT,
+                             SF_rusanovEOS.cpp:141-146                       */
+};
+
+/* Grid description.  Replaces the Field set-up that the reference does
+ * through Field::setup + Mesh::computeMetrics + the boundary applicator
+ * (core/field/SF_field.h:89-96, infrastructure/mesh/SF_mesh.cpp:219-346,
+ * solver/boundary/SF_applicator.h:51-83) for a uniform Cartesian grid. */
+typedef struct {
+    int n[3];        /* unique nodes per axis; 1 on an inactive axis          */
+    int active[3];   /* 1 = axis is discretised (Math::isDirectionActive)     */
+    int bc[3];       /* SF_BC_* for both ends of each active axis             */
+    double d[3];     /* node spacing per axis (1.0 on an inactive axis)       */
+    double gamma;    /* perfect-gas ratio of specific heats                   */
+    int scheme;      /* SF_SCHEME_*                                           */
+    int flux;        /* SF_FLUX_*                                             */
+    int time;        /* SF_TIME_*                                             */
+    int device;      /* HIP device ordinal                                    */
+} sf_grid_desc;
+
+typedef struct sf_grid sf_grid;
+
+/* ---- library ----------------------------------------------------------- */
+int sf_last_error(char *buf, size_t len);
+int sf_version(int *major, int *minor);
+
+/* ---- grid lifetime ----------------------------------------------------- */
+/* Allocates device-resident state (three state buffers, a residual
+ * accumulator and, for RK4, three stage buffers).  Replaces Field::setup
+ * (SF_field.h:94) and PatchWorkspace::ensureFor
+ * (solver/algorithm/SF_patchWorkspace.h:32-44). */
+int sf_grid_create(const sf_grid_desc *desc, sf_grid **out);
+int sf_grid_destroy(sf_grid *grid);
+/* Device bytes held by the grid. */
+int sf_grid_device_bytes(const sf_grid *grid, size_t *bytes);
+
+/* ---- state transfer ------------------------------------------------------ */
+/* Host <-> device copies of the conserved state on the unique nodes
+ * (the reference reads and writes Field::operator(), SF_field.h:271-272). */
+int sf_state_upload(sf_grid *grid, const double *soa);
+int sf_state_download(sf_grid *grid, double *soa);
+/* Same, from / to device memory (a pointer valid on the grid's device). */
+int sf_state_upload_device(sf_grid *grid, const double *soa_device);
+int sf_state_download_device(sf_grid *grid, double *soa_device);
+
+/* ---- the explicit step, operation by operation ---------------------------
+ * These stand in for the op ids that FlowOperations::bindExplicitOps binds
+ * for the "flow.conservative" runtime provider
+ * (solver/algorithm/SF_flowOperations.cpp:466-522,648-652;
+ *  core/system/SF_operationIds.h:29-40). */
+
+/* flow.step.prepare: boundary state at the start of a step
+ * (prepareBoundaryState -> Boundary::Applicator::apply). */
+int sf_step_prepare(sf_grid *grid);
+/* flow.dt.compute: dt = min(CFL scan, max_dt, limit)
+ * (RusanovEOS::deltaT, SF_rusanovEOS.cpp:110-147, then
+ *  SF_flowOperations.cpp:487-497).  The value stays on the device for the
+ * following stages; if dt_out is not NULL it is also copied to the host. */
+int sf_compute_dt(sf_grid *grid, double cfl, double max_dt, double limit,
+                  double *dt_out);
+/* Use a caller-chosen dt for the next step instead of the CFL scan. */
+int sf_set_dt(sf_grid *grid, double dt);
+/* flow.step.begin: Time::Explicit::begin (SF_explicit.cpp:287-321). */
+int sf_step_begin(sf_grid *grid);
+/* explicit.stage.execute: Time::Explicit::executeStage
+ * (SF_explicit.cpp:331-530) with the RHS assembly of
+ * ConservativeRHS::assembleAllPatches (SF_conservativeRHS.cpp:106-247). */
+int sf_stage(sf_grid *grid, int stage);
+/* flow.step.commit + time.commit: boundary state at t+dt, clock advance
+ * (SF_flowOperations.cpp:648-652, SF_singleFluidStepper.cpp:40-46). */
+int sf_step_commit(sf_grid *grid);
+/* Number of stages of the grid's time recipe (1, 3 or 4). */
+int sf_stage_count(const sf_grid *grid, int *count);
+
+/* ---- whole steps --------------------------------------------------------- */
+/* nsteps physical steps with dt = min(CFL dt, max_dt) each, all queued on
+ * the device without host synchronisation between steps.  If dt_seq is not
+ * NULL it receives the nsteps dt values.  Checks the device error flags at
+ * the end and returns SF_E_NUMERIC if any was raised. */
+int sf_run(sf_grid *grid, int nsteps, double cfl, double max_dt,
+           double *dt_seq);
+/* Physical time and step count of the grid's clock. */
+int sf_clock(sf_grid *grid, double *time, long long *steps);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+/* Residual R = Jac * sum_d (F_d(i) - F_d(i-1)) of the current state after
+ * boundary preparation, on the unique nodes (0 on solver-boundary nodes);
+ * Math::localSpatialResidual, SF_residualAssembly.h:12-32. */
+int sf_residual(sf_grid *grid, double *soa_out);
+/* Device error flags (SF_FLAG_*); clear != 0 resets them. */
+int sf_error_flags(sf_grid *grid, uint32_t *flags, int clear);
+/* Wait for the grid's queued work. */
+int sf_synchronize(sf_grid *grid);
+/* The HIP stream (hipStream_t) the grid queues its work on. */
+int sf_grid_stream(sf_grid *grid, void **stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

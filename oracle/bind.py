@@ -1,0 +1,242 @@
+"""ctypes binding of the checker libraries.  TEST INFRASTRUCTURE ONLY.
+
+  liboracle.so      C restatement of the hot path (oracle/sfcpu.c)
+  _ref/libsfref.so  harness over the reference's own translation units
+                    (oracle/ref_harness.cpp); present where it was built
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "liboracle.so"
+REF_SO = HERE / "_ref" / "libsfref.so"
+
+_P = ctypes.POINTER(ctypes.c_double)
+_U = ctypes.POINTER(ctypes.c_uint)
+
+SCHEMES = {"weno5": 0, "teno5": 1}
+FLUXES = {"rusanov": 0, "hllc": 1}
+TIMES = {"euler": 0, "forwardeuler": 0, "ssprk3": 1, "rk4": 2, "classicalrk4": 2}
+BCS = {"transmissive": 0, "periodic": 1}
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 3), ("active", ctypes.c_int * 3),
+                ("bc", ctypes.c_int * 3), ("d", ctypes.c_double * 3),
+                ("gamma", ctypes.c_double), ("scheme", ctypes.c_int),
+                ("flux", ctypes.c_int), ("time", ctypes.c_int)]
+
+
+def make_desc(n, active=None, bc=None, d=(1.0, 1.0, 1.0), gamma=1.4, scheme="weno5",
+              flux="rusanov", time="ssprk3") -> Desc:
+    n = list(n) + [1] * (3 - len(n))
+    active = active if active is not None else [1 if x > 1 else 0 for x in n]
+    bc = bc or ["transmissive"] * 3
+    ds = Desc()
+    ds.n[:] = [int(x) for x in n]
+    ds.active[:] = [int(x) for x in active]
+    ds.bc[:] = [BCS[b] if isinstance(b, str) else int(b) for b in bc]
+    ds.d[:] = [float(x) for x in d]
+    ds.gamma = float(gamma)
+    ds.scheme = SCHEMES[scheme] if isinstance(scheme, str) else int(scheme)
+    ds.flux = FLUXES[flux] if isinstance(flux, str) else int(flux)
+    ds.time = TIMES[time] if isinstance(time, str) else int(time)
+    return ds
+
+
+_oracle = None
+_ref = None
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_SO.exists():
+            build_oracle()
+        lib = ctypes.CDLL(str(ORACLE_SO))
+        lib.sfo_grid_create.restype = ctypes.c_void_p
+        lib.sfo_grid_create.argtypes = [ctypes.POINTER(Desc)]
+        for name in ("sfo_grid_destroy",):
+            getattr(lib, name).argtypes = [ctypes.c_void_p]
+            getattr(lib, name).restype = None
+        for name in ("sfo_set_state", "sfo_get_state", "sfo_residual"):
+            getattr(lib, name).argtypes = [ctypes.c_void_p, _P]
+        lib.sfo_apply_bc.argtypes = [ctypes.c_void_p]
+        lib.sfo_compute_dt.argtypes = [ctypes.c_void_p, ctypes.c_double, _P]
+        lib.sfo_step.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        lib.sfo_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                ctypes.c_double, _P]
+        lib.sfo_error.argtypes = [ctypes.c_void_p]
+        lib.sfo_error.restype = ctypes.c_uint
+        lib.sfo_weno5_core.argtypes = [_P, _U]
+        lib.sfo_weno5_core.restype = ctypes.c_double
+        lib.sfo_teno5_core.argtypes = [_P, _U]
+        lib.sfo_teno5_core.restype = ctypes.c_double
+        lib.sfo_char_matrix.argtypes = [_P, _P, ctypes.c_double, _P, _P]
+        for name in ("sfo_rusanov", "sfo_hllc"):
+            getattr(lib, name).argtypes = [_P, _P, _P, ctypes.c_double, _P]
+        lib.sfo_physical_flux.argtypes = [_P, _P, ctypes.c_double, _P]
+        lib.sfo_char_face_flux.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, _P]
+        lib.sfo_gemv5.argtypes = [_P, _P, _P]
+        _oracle = lib
+    return _oracle
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not REF_SO.exists():
+            raise FileNotFoundError(f"{REF_SO} has not been built (make -C oracle ref)")
+        lib = ctypes.CDLL(str(REF_SO))
+        lib.sfref_run.argtypes = [ctypes.POINTER(Desc), _P, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, _P]
+        lib.sfref_residual.argtypes = [ctypes.POINTER(Desc), _P, _P]
+        lib.sfref_weno5.argtypes = [_P]
+        lib.sfref_weno5.restype = ctypes.c_double
+        lib.sfref_teno5.argtypes = [_P]
+        lib.sfref_teno5.restype = ctypes.c_double
+        lib.sfref_char_matrix.argtypes = [_P, _P, ctypes.c_double, _P, _P]
+        lib.sfref_rusanov.argtypes = [_P, _P, _P, ctypes.c_double, _P]
+        lib.sfref_char_face_flux.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, _P]
+        _ref = lib
+    return _ref
+
+
+def _arr(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+class OracleError(RuntimeError):
+    def __init__(self, flags):
+        super().__init__(f"oracle raised error flags 0x{flags:x}")
+        self.flags = flags
+
+
+def run(desc: Desc, q: np.ndarray, nsteps: int, cfl: float, max_dt: float = 1e300):
+    """Run the C restatement; returns (state, dt sequence)."""
+    lib = oracle()
+    g = lib.sfo_grid_create(ctypes.byref(desc))
+    if not g:
+        raise ValueError("oracle rejected the grid description")
+    try:
+        a = _arr(q).reshape(5, -1).copy()
+        lib.sfo_set_state(g, _ptr(a))
+        dts = np.zeros(max(nsteps, 1))
+        rc = lib.sfo_run(g, nsteps, cfl, max_dt, _ptr(dts))
+        if rc != 0:
+            raise OracleError(lib.sfo_error(g))
+        lib.sfo_get_state(g, _ptr(a))
+        return a, dts[:nsteps]
+    finally:
+        lib.sfo_grid_destroy(g)
+
+
+def residual(desc: Desc, q: np.ndarray):
+    lib = oracle()
+    g = lib.sfo_grid_create(ctypes.byref(desc))
+    try:
+        a = _arr(q).reshape(5, -1).copy()
+        lib.sfo_set_state(g, _ptr(a))
+        out = np.zeros_like(a)
+        if lib.sfo_residual(g, _ptr(out)) != 0:
+            raise OracleError(lib.sfo_error(g))
+        return out
+    finally:
+        lib.sfo_grid_destroy(g)
+
+
+def compute_dt(desc: Desc, q: np.ndarray, cfl: float) -> float:
+    lib = oracle()
+    g = lib.sfo_grid_create(ctypes.byref(desc))
+    try:
+        a = _arr(q).reshape(5, -1).copy()
+        lib.sfo_set_state(g, _ptr(a))
+        dt = ctypes.c_double(0.0)
+        if lib.sfo_compute_dt(g, cfl, ctypes.byref(dt)) != 0:
+            raise OracleError(lib.sfo_error(g))
+        return dt.value
+    finally:
+        lib.sfo_grid_destroy(g)
+
+
+def ref_run(desc: Desc, q: np.ndarray, nsteps: int, cfl: float, max_dt: float = 1e300):
+    """Run the reference's own code through the harness."""
+    a = _arr(q).reshape(5, -1).copy()
+    dts = np.zeros(max(nsteps, 1))
+    ref().sfref_run(ctypes.byref(desc), _ptr(a), nsteps, cfl, max_dt, _ptr(dts))
+    return a, dts[:nsteps]
+
+
+def ref_residual(desc: Desc, q: np.ndarray):
+    a = _arr(q).reshape(5, -1).copy()
+    out = np.zeros_like(a)
+    ref().sfref_residual(ctypes.byref(desc), _ptr(a), _ptr(out))
+    return out
+
+
+# ---- point kernels --------------------------------------------------------
+
+def weno5(v):
+    e = ctypes.c_uint(0)
+    r = oracle().sfo_weno5_core(_ptr(_arr(v)), ctypes.byref(e))
+    return r, e.value
+
+
+def teno5(v):
+    e = ctypes.c_uint(0)
+    r = oracle().sfo_teno5_core(_ptr(_arr(v)), ctypes.byref(e))
+    return r, e.value
+
+
+def char_matrix(q, n, gamma=1.4):
+    L = np.zeros(25)
+    R = np.zeros(25)
+    rc = oracle().sfo_char_matrix(_ptr(_arr(q)), _ptr(_arr(n)), gamma, _ptr(L), _ptr(R))
+    return rc, L.reshape(5, 5), R.reshape(5, 5)
+
+
+def rusanov(qL, qR, n, gamma=1.4):
+    out = np.zeros(5)
+    rc = oracle().sfo_rusanov(_ptr(_arr(qL)), _ptr(_arr(qR)), _ptr(_arr(n)), gamma, _ptr(out))
+    return rc, out
+
+
+def hllc(qL, qR, n, gamma=1.4):
+    out = np.zeros(5)
+    rc = oracle().sfo_hllc(_ptr(_arr(qL)), _ptr(_arr(qR)), _ptr(_arr(n)), gamma, _ptr(out))
+    return rc, out
+
+
+def physical_flux(q, n, gamma=1.4):
+    out = np.zeros(5)
+    rc = oracle().sfo_physical_flux(_ptr(_arr(q)), _ptr(_arr(n)), gamma, _ptr(out))
+    return rc, out
+
+
+def char_face_flux(stencil, cofactor, scheme="weno5", flux="rusanov", gamma=1.4):
+    out = np.zeros(5)
+    rc = oracle().sfo_char_face_flux(_ptr(_arr(stencil).reshape(30)), _ptr(_arr(cofactor)),
+                                     SCHEMES[scheme], FLUXES[flux], gamma, _ptr(out))
+    return rc, out

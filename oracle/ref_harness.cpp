@@ -247,6 +247,18 @@ void export_state(const Case& c, double* soa) {
                         f(i + ng, j + ng, k + ng, v);
 }
 
+// The reference reports orchestration errors by throwing; a C caller gets -1.
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        std::cerr << "[sfref] " << e.what() << std::endl;
+        return -1;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -256,16 +268,16 @@ double sfref_teno5(const double v[5]) { return SF::TENO5::teno5_core(v); }
 
 int sfref_char_matrix(const double q[5], const double n[3], double gamma,
                       double L[25], double R[25]) {
-    SF::Riemann::buildCharacteristicMatrix(q, n, gamma, L, R);
-    return 0;
+    return guarded([&] { SF::Riemann::buildCharacteristicMatrix(q, n, gamma, L, R); });
 }
 
 int sfref_rusanov(const double qL[5], const double qR[5], const double n[3],
                   double gamma, double out[5]) {
-    auto eq = SF::Physics::FluidStateModel::makeSingleFluidPerfectGas(
-        gamma, 287.05, 0.0, 0.72);
-    SF::Numerics::RusanovEOS::faceFlux(*eq, qL, qR, {n[0], n[1], n[2]}, out, 5);
-    return 0;
+    return guarded([&] {
+        auto eq = SF::Physics::FluidStateModel::makeSingleFluidPerfectGas(
+            gamma, 287.05, 0.0, 0.72);
+        SF::Numerics::RusanovEOS::faceFlux(*eq, qL, qR, {n[0], n[1], n[2]}, out, 5);
+    });
 }
 
 int sfref_char_face_flux(const double st[30], const double cof[3], int scheme,
@@ -281,17 +293,19 @@ int sfref_char_face_flux(const double st[30], const double cof[3], int scheme,
         if (flux == SFO_HLLC) hllc_face_flux(qL, qR, n, o, gamma);
         else SF::Numerics::RusanovEOS::faceFlux(*eq, qL, qR, {n[0], n[1], n[2]}, o, 5);
     };
-    if (scheme == SFO_TENO5)
-        SF::Flux::characteristicWENOFaceFlux<6>(stencil, metrics, out,
-                                                SF::TENO5::teno5_core, face, gamma);
-    else
-        SF::Flux::characteristicWENOFaceFlux<6>(stencil, metrics, out,
-                                                SF::WENO5::weno5_core, face, gamma);
-    return 0;
+    return guarded([&] {
+        if (scheme == SFO_TENO5)
+            SF::Flux::characteristicWENOFaceFlux<6>(stencil, metrics, out,
+                                                    SF::TENO5::teno5_core, face, gamma);
+        else
+            SF::Flux::characteristicWENOFaceFlux<6>(stencil, metrics, out,
+                                                    SF::WENO5::weno5_core, face, gamma);
+    });
 }
 
 // Residual of one state after boundary preparation, on the unique nodes.
 int sfref_residual(const sfo_desc* d, const double* soa_in, double* soa_out) {
+  return guarded([&] {
     auto c = make_case(*d, soa_in);
     assemble(*c);
     const SF::Field& f = c->field;
@@ -308,7 +322,7 @@ int sfref_residual(const sfo_desc* d, const double* soa_in, double* soa_out) {
                                                     i + ng, j + ng, k + ng, v)
                         : 0.0;
             }
-    return 0;
+  });
 }
 
 // flow.step.prepare -> flow.dt.compute -> flow.step.begin ->
@@ -316,6 +330,7 @@ int sfref_residual(const sfo_desc* d, const double* soa_in, double* soa_out) {
 // times (SF_flowOperations.cpp:466-522,648-652).
 int sfref_run(const sfo_desc* d, double* soa_io, int nsteps, double cfl,
               double max_dt, double* dt_seq) {
+  return guarded([&] {
     auto c = make_case(*d, soa_io);
     SF::State::StateBundle state;
     state.patches = {&c->field};
@@ -348,7 +363,7 @@ int sfref_run(const sfo_desc* d, double* soa_io, int nsteps, double cfl,
         ++state.step;
     }
     export_state(*c, soa_io);
-    return 0;
+  });
 }
 
 }  // extern "C"

@@ -680,6 +680,38 @@ static int validate(sfo_grid *G)
     return 0;
 }
 
+/* Stage combination of one value (Time::Explicit::applySSPStage /
+ * applyStage / applyFinalRK4 / forwardEuler, SF_explicit.cpp:193-238,
+ * with the tableaux of SF_builtinProviders.cpp:85-127).  q is the current
+ * value, q0 the step-start value, k[0..2] the stored stage residuals of
+ * RK4 and r the residual of this stage (dq/dt = -r). */
+static const double ssp_base[3] = {0.0, 0.75, 1.0 / 3.0};
+static const double ssp_incr[3] = {1.0, 0.25, 2.0 / 3.0};
+static const double rk4_incr[3] = {0.5, 0.5, 1.0};
+static const double rk4_w[4] = {1.0, 2.0, 2.0, 1.0};
+
+double sfo_stage_combine(int time, int stage, double q, double q0,
+                         const double k[3], double r, double dt)
+{
+    if (time == SFO_EULER) return q - dt * r;
+    if (time == SFO_SSPRK3) {
+        const double euler = q - dt * r;
+        return ssp_base[stage] * q0 + ssp_incr[stage] * euler;
+    }
+    if (stage < 3) return q0 - rk4_incr[stage] * dt * r;
+    return q0 - dt / 6.0 * (rk4_w[0] * k[0] + rk4_w[1] * k[1] + rk4_w[2] * k[2]
+                            + rk4_w[3] * r);
+}
+
+/* Stage abscissae c_s (state.time + c_s * dt is the stage time). */
+double sfo_stage_abscissa(int time, int stage)
+{
+    static const double ssp[3] = {0.0, 1.0, 0.5};
+    static const double rk4[4] = {0.0, 0.5, 0.5, 1.0};
+    if (time == SFO_EULER) return 0.0;
+    return time == SFO_SSPRK3 ? ssp[stage] : rk4[stage];
+}
+
 /* One physical step: Time::Explicit::begin/executeStage
  * (SF_explicit.cpp:193-238,287-530) with the tableaux of
  * SF_builtinProviders.cpp:85-127; the boundary state is prepared before
@@ -711,43 +743,19 @@ int sfo_step(sfo_grid *G, double dt)
         size_t c = 5 * at(G, p[0], p[1], p[2]);
         memcpy(G->q0 + c, G->q + c, 5 * sizeof(double));
     }
-    if (scheme == SFO_SSPRK3) {
-        static const double base[3] = {0.0, 0.75, 1.0 / 3.0};
-        static const double incr[3] = {1.0, 0.25, 2.0 / 3.0};
-        for (int st = 0; st < 3; ++st) {
-            if (sfo_apply_bc(G) || assemble_rhs(G, G->k[0])) return -1;
+    if (scheme == SFO_SSPRK3 || scheme == SFO_RK4) {
+        const int stages = scheme == SFO_SSPRK3 ? 3 : 4;
+        for (int st = 0; st < stages; ++st) {
+            double *r = scheme == SFO_RK4 ? G->k[st] : G->k[0];
+            if (sfo_apply_bc(G) || assemble_rhs(G, r)) return -1;
+            if (scheme == SFO_RK4 && validate(G)) return -1;
             FOR_NODES(G, p) {
                 if (!solved(G, p)) continue;
                 size_t c = 5 * at(G, p[0], p[1], p[2]);
                 for (int v = 0; v < 5; ++v) {
-                    const double euler = G->q[c + v] - dt * G->k[0][c + v];
-                    G->q[c + v] = base[st] * G->q0[c + v] + incr[st] * euler;
-                }
-            }
-            if (validate(G)) return -1;
-        }
-        return sfo_apply_bc(G);
-    }
-    if (scheme == SFO_RK4) {
-        static const double incr[3] = {0.5, 0.5, 1.0};
-        static const double w[4] = {1.0, 2.0, 2.0, 1.0};
-        const double divisor = 6.0;
-        for (int st = 0; st < 4; ++st) {
-            if (sfo_apply_bc(G) || assemble_rhs(G, G->k[st])) return -1;
-            if (validate(G)) return -1;
-            FOR_NODES(G, p) {
-                if (!solved(G, p)) continue;
-                size_t c = 5 * at(G, p[0], p[1], p[2]);
-                for (int v = 0; v < 5; ++v) {
-                    if (st < 3) {
-                        G->q[c + v] = G->q0[c + v] - incr[st] * dt * G->k[st][c + v];
-                    } else {
-                        G->q[c + v] = G->q0[c + v]
-                            - dt / divisor * (w[0] * G->k[0][c + v]
-                                              + w[1] * G->k[1][c + v]
-                                              + w[2] * G->k[2][c + v]
-                                              + w[3] * G->k[3][c + v]);
-                    }
+                    const double k[3] = {G->k[0][c + v], G->k[1][c + v], G->k[2][c + v]};
+                    G->q[c + v] = sfo_stage_combine(scheme, st, G->q[c + v], G->q0[c + v],
+                                                    k, r[c + v], dt);
                 }
             }
             if (validate(G)) return -1;

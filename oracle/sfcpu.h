@@ -61,6 +61,11 @@ int sfo_physical_flux(const double q[5], const double n[3], double gamma,
 int sfo_char_face_flux(const double stencil[30], const double cofactor[3],
                        int scheme, int flux, double gamma, double out[5]);
 
+/* stage combination of one value, dq/dt = -r (see sfcpu.c) */
+double sfo_stage_combine(int time, int stage, double q, double q0,
+                         const double k[3], double r, double dt);
+double sfo_stage_abscissa(int time, int stage);
+
 /* ---- grid ------------------------------------------------------------- */
 typedef struct sfo_grid sfo_grid;
 

@@ -1,0 +1,238 @@
+"""ctypes binding of libsonic_amd.so (the C ABI in include/sonic_amd.h).
+
+The shared library is built in-tree (``__graft_entry__.build()`` or
+``python -m sonicsolver_amd.build``).  Loading fails loudly when it is
+missing: there is no CPU fallback for any operation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libsonic_amd.so"
+
+SCHEMES = {"weno5": 0, "teno5": 1}
+FLUXES = {"rusanov": 0, "hllc": 1}
+TIMES = {"forwardeuler": 0, "euler": 0, "ssprk3": 1, "classicalrk4": 2, "rk4": 2}
+BCS = {"transmissive": 0, "zerogradient": 0, "periodic": 1}
+
+FLAG_NAMES = {1: "state", 2: "face-average", 4: "teno-cutoff", 8: "flux", 16: "dt"}
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int * 3),
+        ("active", ctypes.c_int * 3),
+        ("bc", ctypes.c_int * 3),
+        ("d", ctypes.c_double * 3),
+        ("gamma", ctypes.c_double),
+        ("scheme", ctypes.c_int),
+        ("flux", ctypes.c_int),
+        ("time", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+class SonicError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+_lib = None
+
+_P = ctypes.POINTER(ctypes.c_double)
+_VP = ctypes.c_void_p
+
+
+def _declare(lib):
+    sig = {
+        "sf_last_error": [ctypes.c_char_p, ctypes.c_size_t],
+        "sf_version": [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+        "sf_grid_create": [ctypes.POINTER(GridDesc), ctypes.POINTER(_VP)],
+        "sf_grid_destroy": [_VP],
+        "sf_grid_device_bytes": [_VP, ctypes.POINTER(ctypes.c_size_t)],
+        "sf_grid_stream": [_VP, ctypes.POINTER(_VP)],
+        "sf_state_upload": [_VP, _P],
+        "sf_state_download": [_VP, _P],
+        "sf_state_upload_device": [_VP, _VP],
+        "sf_state_download_device": [_VP, _VP],
+        "sf_step_prepare": [_VP],
+        "sf_compute_dt": [_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P],
+        "sf_set_dt": [_VP, ctypes.c_double],
+        "sf_step_begin": [_VP],
+        "sf_stage": [_VP, ctypes.c_int],
+        "sf_step_commit": [_VP],
+        "sf_stage_count": [_VP, ctypes.POINTER(ctypes.c_int)],
+        "sf_run": [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P],
+        "sf_clock": [_VP, _P, ctypes.POINTER(ctypes.c_longlong)],
+        "sf_residual": [_VP, _P],
+        "sf_error_flags": [_VP, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int],
+        "sf_synchronize": [_VP],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    return lib
+
+
+def load():
+    """Load libsonic_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m sonicsolver_amd.build` "
+                "(there is no CPU fallback)")
+        _lib = _declare(ctypes.CDLL(str(LIB_PATH)))
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        buf = ctypes.create_string_buffer(1024)
+        load().sf_last_error(buf, len(buf))
+        raise SonicError(rc, buf.value.decode(errors="replace"))
+
+
+def make_desc(n, active=None, bc=None, d=(1.0, 1.0, 1.0), gamma=1.4, scheme="weno5",
+              flux="rusanov", time="ssprk3", device=0) -> GridDesc:
+    n = list(n) + [1] * (3 - len(n))
+    if active is None:
+        active = [1 if x > 1 else 0 for x in n]
+    bc = bc or ["transmissive"] * 3
+    desc = GridDesc()
+    desc.n[:] = [int(x) for x in n]
+    desc.active[:] = [int(x) for x in active]
+    desc.bc[:] = [BCS[b] if isinstance(b, str) else int(b) for b in bc]
+    desc.d[:] = [float(x) for x in d]
+    desc.gamma = float(gamma)
+    desc.scheme = SCHEMES[scheme.lower()] if isinstance(scheme, str) else int(scheme)
+    desc.flux = FLUXES[flux.lower()] if isinstance(flux, str) else int(flux)
+    desc.time = TIMES[time.lower()] if isinstance(time, str) else int(time)
+    desc.device = int(device)
+    return desc
+
+
+class Grid:
+    """A device-resident structured grid (one sf_grid handle)."""
+
+    def __init__(self, desc: GridDesc):
+        lib = load()
+        self.desc = desc
+        self.n = tuple(desc.n)
+        self.N = int(np.prod(self.n))
+        h = _VP()
+        _check(lib.sf_grid_create(ctypes.byref(desc), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().sf_grid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def upload(self, soa: np.ndarray):
+        a = np.ascontiguousarray(soa, dtype=np.float64).reshape(5, self.N)
+        _check(load().sf_state_upload(self._h, a.ctypes.data_as(_P)))
+
+    def download(self) -> np.ndarray:
+        out = np.empty((5, self.N), dtype=np.float64)
+        _check(load().sf_state_download(self._h, out.ctypes.data_as(_P)))
+        return out
+
+    def upload_device(self, ptr: int):
+        _check(load().sf_state_upload_device(self._h, _VP(ptr)))
+
+    def download_device(self, ptr: int):
+        _check(load().sf_state_download_device(self._h, _VP(ptr)))
+
+    def run(self, nsteps: int, cfl: float, max_dt: float = 1.0e300, record_dt: bool = True):
+        dts = np.zeros(max(nsteps, 1), dtype=np.float64)
+        _check(load().sf_run(self._h, int(nsteps), float(cfl), float(max_dt),
+                             dts.ctypes.data_as(_P) if record_dt else None))
+        return dts[:nsteps]
+
+    def residual(self) -> np.ndarray:
+        out = np.empty((5, self.N), dtype=np.float64)
+        _check(load().sf_residual(self._h, out.ctypes.data_as(_P)))
+        return out
+
+    def prepare(self):
+        _check(load().sf_step_prepare(self._h))
+
+    def compute_dt(self, cfl: float, max_dt: float = 1.0e300, limit: float = 1.0e300) -> float:
+        dt = ctypes.c_double(0.0)
+        _check(load().sf_compute_dt(self._h, cfl, max_dt, limit, ctypes.byref(dt)))
+        return dt.value
+
+    def set_dt(self, dt: float):
+        _check(load().sf_set_dt(self._h, float(dt)))
+
+    def begin(self):
+        _check(load().sf_step_begin(self._h))
+
+    def stage(self, s: int):
+        _check(load().sf_stage(self._h, int(s)))
+
+    def commit(self):
+        _check(load().sf_step_commit(self._h))
+
+    def stage_count(self) -> int:
+        c = ctypes.c_int(0)
+        _check(load().sf_stage_count(self._h, ctypes.byref(c)))
+        return c.value
+
+    def clock(self):
+        t = ctypes.c_double(0.0)
+        s = ctypes.c_longlong(0)
+        _check(load().sf_clock(self._h, ctypes.byref(t), ctypes.byref(s)))
+        return t.value, s.value
+
+    def flags(self, clear: bool = False) -> int:
+        f = ctypes.c_uint32(0)
+        _check(load().sf_error_flags(self._h, ctypes.byref(f), int(clear)))
+        return f.value
+
+    def synchronize(self):
+        _check(load().sf_synchronize(self._h))
+
+    def stream(self) -> int:
+        s = _VP()
+        _check(load().sf_grid_stream(self._h, ctypes.byref(s)))
+        return s.value or 0
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_size_t(0)
+        _check(load().sf_grid_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+
+def describe_flags(flags: int) -> str:
+    names = [v for k, v in FLAG_NAMES.items() if flags & k]
+    return ",".join(names) if names else "none"
+
+
+def env_device() -> int:
+    return int(os.environ.get("SONIC_AMD_DEVICE", "0"))

@@ -1,0 +1,47 @@
+"""Build libsonic_amd.so in-tree for gfx950 with hipcc.
+
+    python -m sonicsolver_amd.build
+
+The HIP sources are compiled with packed FP32 instructions disabled
+(``-Xclang -target-feature -Xclang -packed-fp32-ops``); hipcc's host pass
+warns that it does not know that feature, which is harmless.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("SONIC_AMD_ARCH", "gfx950")
+
+SOURCES = [PKG / "csrc" / "sf_capi.hip", PKG / "host" / "sf_provider.cpp"]
+HEADERS = list((PKG / "csrc").glob("*.h")) + list((PKG / "host").glob("*.h")) + \
+    list((ROOT / "include").glob("*.h"))
+OUTPUT = PKG / "libsonic_amd.so"
+
+
+def hip_flags():
+    return [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            f"-I{ROOT / 'include'}", f"-I{PKG / 'csrc'}", f"-I{PKG / 'host'}"]
+
+
+def build(verbose: bool = False) -> Path:
+    srcs = [s for s in SOURCES if s.exists()]
+    cmd = [HIPCC, *hip_flags(), "-shared", "-o", str(OUTPUT) + ".tmp", *map(str, srcs)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed with status {res.returncode}")
+    os.replace(str(OUTPUT) + ".tmp", OUTPUT)
+    return OUTPUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,638 @@
+// sf_capi.hip -- host side of the C ABI declared in include/sonic_amd.h:
+// device buffers, the per-stage launch sequence and the error handling.
+#include "sonic_amd.h"
+#include "sf_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+
+using namespace sfamd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define SF_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return fail(SF_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kMarchSeg = 32;
+constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
+
+}  // namespace
+
+struct sf_grid {
+    sf_grid_desc desc{};
+    GridDev G{};
+    hipStream_t stream = nullptr;
+    double* buf[3] = {nullptr, nullptr, nullptr};
+    double* S = nullptr;
+    double* k[3] = {nullptr, nullptr, nullptr};
+    Clock* clock = nullptr;
+    unsigned* flags = nullptr;
+    double* dt_hist = nullptr;
+    int dt_hist_cap = 0;
+    double* stage_io = nullptr;  // unique-node staging for transfers
+    size_t bytes = 0;
+    int cur = 0;                 // buffer holding the current state
+    int ghosts_valid = 0;        // ghost layers of buf[cur] are up to date
+    int dtmin_valid = 0;         // clock->dtmin holds the scan of buf[cur]
+    double dtmin_cfl = 0.0;      // CFL number the folded scan used
+    int step_open = 0;
+    int next_stage = 0;
+    int stage_in[4] = {0, 0, 0, 0}, stage_out[4] = {0, 0, 0, 0};
+    double step_cfl = 0.5;       // CFL folded into the last stage's update
+    int fold_cfl = 0;
+    double max_dt = std::numeric_limits<double>::max();
+    double limit = std::numeric_limits<double>::max();
+};
+
+namespace {
+
+int nstages(int time) { return time == SF_TIME_EULER ? 1 : time == SF_TIME_SSPRK3 ? 3 : 4; }
+
+int check_desc(const sf_grid_desc* d) {
+    if (!d) return fail(SF_E_ARG, "sf_grid_create: null description");
+    int nact = 0;
+    for (int a = 0; a < 3; ++a) {
+        if (d->active[a] != 0 && d->active[a] != 1)
+            return fail(SF_E_ARG, "active[] entries must be 0 or 1");
+        if (d->active[a]) {
+            ++nact;
+            if (d->n[a] < 4) return fail(SF_E_ARG, "an active axis needs at least 4 nodes");
+            if (d->bc[a] != SF_BC_TRANSMISSIVE && d->bc[a] != SF_BC_PERIODIC)
+                return fail(SF_E_ARG, "unknown boundary kind");
+        } else if (d->n[a] != 1) {
+            return fail(SF_E_ARG, "an inactive axis must have exactly one node");
+        }
+        if (!(d->d[a] > 0.0) || !std::isfinite(d->d[a]))
+            return fail(SF_E_ARG, "node spacing must be positive and finite");
+    }
+    if (nact == 0) return fail(SF_E_ARG, "at least one axis must be active");
+    if (!d->active[0]) return fail(SF_E_ARG, "the x axis must be active");
+    if (d->active[2] && !d->active[1])
+        return fail(SF_E_ARG, "a z-active grid must also be y-active");
+    if (!(d->gamma > 1.0) || !std::isfinite(d->gamma)) return fail(SF_E_ARG, "gamma must exceed 1");
+    if (d->scheme != SF_SCHEME_WENO5 && d->scheme != SF_SCHEME_TENO5)
+        return fail(SF_E_ARG, "unknown reconstruction scheme");
+    if (d->flux != SF_FLUX_RUSANOV && d->flux != SF_FLUX_HLLC)
+        return fail(SF_E_ARG, "unknown face flux");
+    if (d->time != SF_TIME_EULER && d->time != SF_TIME_SSPRK3 && d->time != SF_TIME_RK4)
+        return fail(SF_E_ARG, "unknown time recipe");
+    const long long N = (long long)d->n[0] * d->n[1] * d->n[2];
+    if (N > (1LL << 31)) return fail(SF_E_ARG, "grid exceeds 2^31 nodes");
+    return 0;
+}
+
+unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// Ghost fill in the order of Boundary::Applicator::apply (density and
+// momentum for every transmissive set, then energy from pressure, axis by
+// axis), then the periodic axes.
+int fill_ghosts(sf_grid* g, double* q) {
+    const GridDev& G = g->G;
+    auto plane = [&](int a) {
+        return 2LL * G.n[(a + 1) % 3] * G.n[(a + 2) % 3];
+    };
+    for (int a = 0; a < 3; ++a) {
+        if (!G.active[a] || G.bc[a] != 0) continue;
+        const unsigned nb = blocks_for(plane(a), 256);
+        if (a == 0) k_bc_copy4<0><<<nb, 256, 0, g->stream>>>(G, q);
+        if (a == 1) k_bc_copy4<1><<<nb, 256, 0, g->stream>>>(G, q);
+        if (a == 2) k_bc_copy4<2><<<nb, 256, 0, g->stream>>>(G, q);
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (!G.active[a] || G.bc[a] != 0) continue;
+        const unsigned nb = blocks_for(plane(a), 256);
+        if (a == 0) k_bc_energy<0><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
+        if (a == 1) k_bc_energy<1><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
+        if (a == 2) k_bc_energy<2><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (!G.active[a] || G.bc[a] != 1) continue;
+        const unsigned nb = blocks_for(plane(a), 256);
+        if (a == 0) k_bc_periodic<0><<<nb, 256, 0, g->stream>>>(G, q);
+        if (a == 1) k_bc_periodic<1><<<nb, 256, 0, g->stream>>>(G, q);
+        if (a == 2) k_bc_periodic<2><<<nb, 256, 0, g->stream>>>(G, q);
+    }
+    SF_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int SCHEME, int FLUX, int MODE>
+void launch_axis(sf_grid* g, int axis, const double* q, const StageArgs& A) {
+    const GridDev& G = g->G;
+    if (axis == 0) {
+        dim3 grid(blocks_for(G.n[0], kXBlock - 1), G.n[1], G.n[2]);
+        k_sweep_x<SCHEME, FLUX, MODE><<<grid, kXBlock, 0, g->stream>>>(G, q, g->S, A);
+        return;
+    }
+    const int other = axis == 1 ? G.n[2] : G.n[1];
+    dim3 grid(blocks_for(G.n[0], kMarchBlock), other, blocks_for(G.n[axis], kMarchSeg));
+    if (axis == 1)
+        k_sweep_march<1, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, g->stream>>>(G, q, g->S, A,
+                                                                                 kMarchSeg);
+    else
+        k_sweep_march<2, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, g->stream>>>(G, q, g->S, A,
+                                                                                 kMarchSeg);
+}
+
+template <int SCHEME, int FLUX>
+void launch_sweeps(sf_grid* g, const double* q, const StageArgs& A) {
+    int axes[3], na = 0;
+    for (int a = 0; a < 3; ++a)
+        if (g->G.active[a]) axes[na++] = a;
+    for (int t = 0; t < na; ++t) {
+        if (t == na - 1) launch_axis<SCHEME, FLUX, kFinal>(g, axes[t], q, A);
+        else if (t == 0) launch_axis<SCHEME, FLUX, kCopyS>(g, axes[t], q, A);
+        else launch_axis<SCHEME, FLUX, kAccumS>(g, axes[t], q, A);
+    }
+}
+
+int run_sweeps(sf_grid* g, const double* q, const StageArgs& A) {
+    const int s = g->desc.scheme, f = g->desc.flux;
+    if (s == 0 && f == 0) launch_sweeps<0, 0>(g, q, A);
+    else if (s == 0 && f == 1) launch_sweeps<0, 1>(g, q, A);
+    else if (s == 1 && f == 0) launch_sweeps<1, 0>(g, q, A);
+    else launch_sweeps<1, 1>(g, q, A);
+    SF_HIP(hipGetLastError());
+    return 0;
+}
+
+int set_device(const sf_grid* g) {
+    SF_HIP(hipSetDevice(g->desc.device));
+    return 0;
+}
+
+int ensure_hist(sf_grid* g, int n) {
+    if (n <= g->dt_hist_cap) return 0;
+    if (g->dt_hist) SF_HIP(hipFree(g->dt_hist));
+    g->dt_hist = nullptr;
+    SF_HIP(hipMalloc(&g->dt_hist, sizeof(double) * (size_t)n));
+    g->dt_hist_cap = n;
+    return 0;
+}
+
+int check_flags(sf_grid* g, const char* where) {
+    unsigned h = 0;
+    SF_HIP(hipMemcpyAsync(&h, g->flags, sizeof h, hipMemcpyDeviceToHost, g->stream));
+    SF_HIP(hipStreamSynchronize(g->stream));
+    if (h) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "%s: device raised numerical fault flags 0x%x "
+                      "(1 state, 2 face average, 4 TENO cutoff, 8 flux, 16 dt)",
+                      where, h);
+        return fail(SF_E_NUMERIC, msg);
+    }
+    return 0;
+}
+
+// The next step's CFL scan: folded into the last stage's update when
+// possible, otherwise a standalone pass.
+int scan_dt(sf_grid* g, double cfl) {
+    if (!(g->dtmin_valid && g->dtmin_cfl == cfl)) {
+        k_set_limits<<<1, 64, 0, g->stream>>>(g->clock, g->max_dt, g->limit, 1);
+        const long long N = (long long)g->G.n[0] * g->G.n[1] * g->G.n[2];
+        const unsigned nb = std::min<unsigned>(blocks_for(N, 256), 8192u);
+        k_cfl<<<nb, 256, 0, g->stream>>>(g->G, g->buf[g->cur], cfl, &g->clock->dtmin, g->flags);
+        SF_HIP(hipGetLastError());
+    }
+    g->dtmin_valid = 0;
+    return 0;
+}
+
+int set_limits(sf_grid* g, double max_dt, double limit) {
+    g->max_dt = max_dt;
+    g->limit = limit;
+    k_set_limits<<<1, 64, 0, g->stream>>>(g->clock, max_dt, limit, 0);
+    SF_HIP(hipGetLastError());
+    return 0;
+}
+
+int do_stage(sf_grid* g, int s, int fold_dt) {
+    const int in = g->stage_in[s], out = g->stage_out[s];
+    if (s == 0 && !g->ghosts_valid) {
+        if (int rc = fill_ghosts(g, g->buf[in])) return rc;
+    }
+    if (s > 0) {
+        if (int rc = fill_ghosts(g, g->buf[in])) return rc;
+    }
+    StageArgs A{};
+    A.q0 = g->buf[g->cur];
+    A.qout = g->buf[out];
+    for (int i = 0; i < 3; ++i) A.k[i] = g->k[i];
+    A.dt = &g->clock->dt;
+    A.dtmin = &g->clock->dtmin;
+    A.flags = g->flags;
+    A.time = g->desc.time;
+    A.stage = s;
+    if (g->desc.time == SF_TIME_SSPRK3) {
+        static const double base[3] = {0.0, 0.75, 1.0 / 3.0};
+        static const double incr[3] = {1.0, 0.25, 2.0 / 3.0};
+        A.base = base[s];
+        A.incr = incr[s];
+    } else if (g->desc.time == SF_TIME_RK4) {
+        static const double incr[4] = {0.5, 0.5, 1.0, 0.0};
+        A.incr = incr[s];
+    } else {
+        A.incr = 1.0;
+    }
+    const bool last = s == nstages(g->desc.time) - 1;
+    A.reduce_dt = (last && fold_dt) ? 1 : 0;
+    A.cfl = g->step_cfl;
+    return run_sweeps(g, g->buf[in], A);
+}
+
+void plan_buffers(sf_grid* g) {
+    const int a = g->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+    switch (g->desc.time) {
+    case SF_TIME_EULER:
+        g->stage_in[0] = a; g->stage_out[0] = b;
+        break;
+    case SF_TIME_SSPRK3:
+        g->stage_in[0] = a; g->stage_out[0] = b;
+        g->stage_in[1] = b; g->stage_out[1] = c;
+        g->stage_in[2] = c; g->stage_out[2] = a;
+        break;
+    default:
+        g->stage_in[0] = a; g->stage_out[0] = b;
+        g->stage_in[1] = b; g->stage_out[1] = c;
+        g->stage_in[2] = c; g->stage_out[2] = b;
+        g->stage_in[3] = b; g->stage_out[3] = a;
+        break;
+    }
+}
+
+int grid_free(sf_grid* g) {
+    if (g->stream) hipStreamSynchronize(g->stream);
+    for (double* p : g->buf) if (p) hipFree(p);
+    for (double* p : g->k) if (p) hipFree(p);
+    if (g->S) hipFree(g->S);
+    if (g->clock) hipFree(g->clock);
+    if (g->flags) hipFree(g->flags);
+    if (g->dt_hist) hipFree(g->dt_hist);
+    if (g->stage_io) hipFree(g->stage_io);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sf_last_error(char* buf, size_t len) {
+    if (!buf || len == 0) return SF_E_ARG;
+    std::snprintf(buf, len, "%s", g_last_error.c_str());
+    return 0;
+}
+
+int sf_version(int* major, int* minor) {
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    return 0;
+}
+
+int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
+    if (!out) return fail(SF_E_ARG, "sf_grid_create: null output handle");
+    *out = nullptr;
+    if (int rc = check_desc(desc)) return rc;
+    int ndev = 0;
+    SF_HIP(hipGetDeviceCount(&ndev));
+    if (desc->device < 0 || desc->device >= ndev)
+        return fail(SF_E_ARG, "sf_grid_create: device ordinal out of range");
+    SF_HIP(hipSetDevice(desc->device));
+    sf_grid* g = new (std::nothrow) sf_grid;
+    if (!g) return fail(SF_E_MEMORY, "sf_grid_create: host allocation failed");
+    g->desc = *desc;
+    GridDev& G = g->G;
+    for (int a = 0; a < 3; ++a) {
+        G.n[a] = desc->n[a];
+        G.g[a] = desc->active[a] ? 3 : 0;
+        G.m[a] = G.n[a] + 2 * G.g[a];
+        G.active[a] = desc->active[a];
+        G.bc[a] = desc->bc[a];
+    }
+    G.stride[0] = 1;
+    G.stride[1] = G.m[0];
+    G.stride[2] = (long long)G.m[0] * G.m[1];
+    G.P = (long long)G.m[0] * G.m[1] * G.m[2];
+    G.gamma = desc->gamma;
+    // Uniform Cartesian metrics exactly as the reference forms them:
+    // Jac = 1/J, metric = 1/d, face cofactor = 0.5*(metric/Jac + metric/Jac),
+    // area = |cofactor| (SF_mesh.cpp:320-338, SF_canonicalFace.h:87-108).
+    G.jac = 1.0 / (desc->d[0] * desc->d[1] * desc->d[2]);
+    for (int a = 0; a < 3; ++a) {
+        G.metric[a] = 1.0 / desc->d[a];
+        const double c = G.metric[a] / G.jac;
+        const double cof = 0.5 * (c + c);
+        G.area[a] = std::sqrt(cof * cof + 0.0 * 0.0 + 0.0 * 0.0);
+    }
+    const size_t state = sizeof(double) * 5 * (size_t)G.P;
+    const size_t N = (size_t)desc->n[0] * desc->n[1] * desc->n[2];
+    auto alloc = [&](double** p, size_t bytes) -> int {
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return fail(SF_E_MEMORY, std::string("sf_grid_create: hipMalloc of ") +
+                                         std::to_string(bytes) + " bytes failed: " +
+                                         hipGetErrorString(e));
+        }
+        g->bytes += bytes;
+        return 0;
+    };
+    int rc = 0;
+    for (int i = 0; i < 3 && !rc; ++i) rc = alloc(&g->buf[i], state);
+    if (!rc) rc = alloc(&g->S, state);
+    if (!rc && desc->time == SF_TIME_RK4)
+        for (int i = 0; i < 3 && !rc; ++i) rc = alloc(&g->k[i], state);
+    if (!rc) rc = alloc(&g->stage_io, sizeof(double) * 5 * N);
+    if (!rc && hipMalloc(&g->clock, sizeof(Clock)) != hipSuccess)
+        rc = fail(SF_E_MEMORY, "sf_grid_create: clock allocation failed");
+    if (!rc && hipMalloc(&g->flags, sizeof(unsigned)) != hipSuccess)
+        rc = fail(SF_E_MEMORY, "sf_grid_create: flag allocation failed");
+    if (!rc && hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(SF_E_HIP, "sf_grid_create: stream creation failed");
+    if (rc) {
+        grid_free(g);
+        return rc;
+    }
+    Clock c{};
+    c.dt = 0.0;
+    c.time = 0.0;
+    c.steps = 0;
+    c.dtmin = kInfBits;
+    c.max_dt = std::numeric_limits<double>::max();
+    c.limit = std::numeric_limits<double>::max();
+    unsigned zero = 0;
+    if (hipMemcpy(g->clock, &c, sizeof c, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->flags, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(g->buf[0], 0, state) != hipSuccess ||
+        hipMemset(g->buf[1], 0, state) != hipSuccess ||
+        hipMemset(g->buf[2], 0, state) != hipSuccess ||
+        hipMemset(g->S, 0, state) != hipSuccess) {
+        grid_free(g);
+        return fail(SF_E_HIP, "sf_grid_create: initialisation copy failed");
+    }
+    *out = g;
+    return 0;
+}
+
+int sf_grid_destroy(sf_grid* g) {
+    if (!g) return 0;
+    set_device(g);
+    return grid_free(g);
+}
+
+int sf_grid_device_bytes(const sf_grid* g, size_t* bytes) {
+    if (!g || !bytes) return fail(SF_E_ARG, "sf_grid_device_bytes: null argument");
+    *bytes = g->bytes;
+    return 0;
+}
+
+int sf_grid_stream(sf_grid* g, void** stream) {
+    if (!g || !stream) return fail(SF_E_ARG, "sf_grid_stream: null argument");
+    *stream = (void*)g->stream;
+    return 0;
+}
+
+static int upload_from(sf_grid* g, const double* dsrc) {
+    const long long N = (long long)g->G.n[0] * g->G.n[1] * g->G.n[2];
+    const unsigned nb = std::min<unsigned>(blocks_for(N, 256), 65535u);
+    k_pack<<<nb, 256, 0, g->stream>>>(g->G, dsrc, g->buf[g->cur], 1);
+    SF_HIP(hipGetLastError());
+    g->ghosts_valid = 0;
+    g->dtmin_valid = 0;
+    g->step_open = 0;
+    return 0;
+}
+
+int sf_state_upload(sf_grid* g, const double* soa) {
+    if (!g || !soa) return fail(SF_E_ARG, "sf_state_upload: null argument");
+    if (int rc = set_device(g)) return rc;
+    const size_t N = (size_t)g->G.n[0] * g->G.n[1] * g->G.n[2];
+    SF_HIP(hipMemcpyAsync(g->stage_io, soa, sizeof(double) * 5 * N, hipMemcpyHostToDevice,
+                          g->stream));
+    if (int rc = upload_from(g, g->stage_io)) return rc;
+    SF_HIP(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int sf_state_upload_device(sf_grid* g, const double* soa_device) {
+    if (!g || !soa_device) return fail(SF_E_ARG, "sf_state_upload_device: null argument");
+    if (int rc = set_device(g)) return rc;
+    return upload_from(g, soa_device);
+}
+
+static int download_to(sf_grid* g, double* ddst) {
+    const long long N = (long long)g->G.n[0] * g->G.n[1] * g->G.n[2];
+    const unsigned nb = std::min<unsigned>(blocks_for(N, 256), 65535u);
+    k_pack<<<nb, 256, 0, g->stream>>>(g->G, g->buf[g->cur], ddst, 0);
+    SF_HIP(hipGetLastError());
+    return 0;
+}
+
+int sf_state_download(sf_grid* g, double* soa) {
+    if (!g || !soa) return fail(SF_E_ARG, "sf_state_download: null argument");
+    if (int rc = set_device(g)) return rc;
+    if (int rc = download_to(g, g->stage_io)) return rc;
+    const size_t N = (size_t)g->G.n[0] * g->G.n[1] * g->G.n[2];
+    SF_HIP(hipMemcpyAsync(soa, g->stage_io, sizeof(double) * 5 * N, hipMemcpyDeviceToHost,
+                          g->stream));
+    SF_HIP(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+int sf_state_download_device(sf_grid* g, double* soa_device) {
+    if (!g || !soa_device) return fail(SF_E_ARG, "sf_state_download_device: null argument");
+    if (int rc = set_device(g)) return rc;
+    return download_to(g, soa_device);
+}
+
+int sf_step_prepare(sf_grid* g) {
+    if (!g) return fail(SF_E_ARG, "sf_step_prepare: null grid");
+    if (int rc = set_device(g)) return rc;
+    if (g->step_open) return fail(SF_E_STATE, "sf_step_prepare: a step is open");
+    if (!g->ghosts_valid) {
+        if (int rc = fill_ghosts(g, g->buf[g->cur])) return rc;
+        g->ghosts_valid = 1;
+    }
+    return 0;
+}
+
+int sf_compute_dt(sf_grid* g, double cfl, double max_dt, double limit, double* dt_out) {
+    if (!g) return fail(SF_E_ARG, "sf_compute_dt: null grid");
+    if (!(cfl > 0.0) || !std::isfinite(cfl)) return fail(SF_E_ARG, "sf_compute_dt: invalid CFL");
+    if (!(max_dt > 0.0) || !(limit > 0.0))
+        return fail(SF_E_ARG, "sf_compute_dt: limits must be positive");
+    if (int rc = set_device(g)) return rc;
+    if (g->step_open) return fail(SF_E_STATE, "sf_compute_dt: a step is open");
+    if (int rc = set_limits(g, max_dt, limit)) return rc;
+    if (int rc = scan_dt(g, cfl)) return rc;
+    k_finalize_dt<<<1, 64, 0, g->stream>>>(g->clock, nullptr, 0);
+    SF_HIP(hipGetLastError());
+    g->step_cfl = cfl;
+    if (dt_out) {
+        SF_HIP(hipMemcpyAsync(dt_out, &g->clock->dt, sizeof(double), hipMemcpyDeviceToHost,
+                              g->stream));
+        SF_HIP(hipStreamSynchronize(g->stream));
+        if (int rc = check_flags(g, "sf_compute_dt")) return rc;
+    }
+    return 0;
+}
+
+int sf_set_dt(sf_grid* g, double dt) {
+    if (!g) return fail(SF_E_ARG, "sf_set_dt: null grid");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(SF_E_ARG, "sf_set_dt: invalid dt");
+    if (int rc = set_device(g)) return rc;
+    if (g->step_open) return fail(SF_E_STATE, "sf_set_dt: a step is open");
+    k_set_dt<<<1, 64, 0, g->stream>>>(g->clock, dt);
+    SF_HIP(hipGetLastError());
+    return 0;
+}
+
+int sf_step_begin(sf_grid* g) {
+    if (!g) return fail(SF_E_ARG, "sf_step_begin: null grid");
+    if (g->step_open) return fail(SF_E_STATE, "sf_step_begin: a step is already open");
+    plan_buffers(g);
+    g->step_open = 1;
+    g->next_stage = 0;
+    return 0;
+}
+
+int sf_stage_count(const sf_grid* g, int* count) {
+    if (!g || !count) return fail(SF_E_ARG, "sf_stage_count: null argument");
+    *count = nstages(g->desc.time);
+    return 0;
+}
+
+int sf_stage(sf_grid* g, int stage) {
+    if (!g) return fail(SF_E_ARG, "sf_stage: null grid");
+    if (!g->step_open || stage != g->next_stage || stage >= nstages(g->desc.time))
+        return fail(SF_E_STATE, "sf_stage: stage out of order or no open step");
+    if (int rc = set_device(g)) return rc;
+    if (int rc = do_stage(g, stage, g->fold_cfl)) return rc;
+    g->next_stage++;
+    return 0;
+}
+
+int sf_step_commit(sf_grid* g) {
+    if (!g) return fail(SF_E_ARG, "sf_step_commit: null grid");
+    if (!g->step_open || g->next_stage != nstages(g->desc.time))
+        return fail(SF_E_STATE, "sf_step_commit: stages incomplete");
+    if (int rc = set_device(g)) return rc;
+    g->cur = g->stage_out[nstages(g->desc.time) - 1];
+    if (int rc = fill_ghosts(g, g->buf[g->cur])) return rc;
+    k_commit_clock<<<1, 64, 0, g->stream>>>(g->clock);
+    SF_HIP(hipGetLastError());
+    g->ghosts_valid = 1;
+    g->dtmin_valid = g->fold_cfl;
+    g->dtmin_cfl = g->step_cfl;
+    g->step_open = 0;
+    return 0;
+}
+
+int sf_run(sf_grid* g, int nsteps, double cfl, double max_dt, double* dt_seq) {
+    if (!g) return fail(SF_E_ARG, "sf_run: null grid");
+    if (nsteps < 0) return fail(SF_E_ARG, "sf_run: negative step count");
+    if (!(cfl > 0.0) || !std::isfinite(cfl)) return fail(SF_E_ARG, "sf_run: invalid CFL");
+    if (!(max_dt > 0.0)) return fail(SF_E_ARG, "sf_run: max_dt must be positive");
+    if (g->step_open) return fail(SF_E_STATE, "sf_run: a step is open");
+    if (int rc = set_device(g)) return rc;
+    if (dt_seq && nsteps > 0) {
+        if (int rc = ensure_hist(g, nsteps)) return rc;
+    }
+    if (int rc = set_limits(g, max_dt, std::numeric_limits<double>::max())) return rc;
+    g->fold_cfl = 1;
+    g->step_cfl = cfl;
+    for (int s = 0; s < nsteps; ++s) {
+        if (int rc = sf_step_prepare(g)) return rc;
+        if (int rc = scan_dt(g, cfl)) return rc;
+        k_finalize_dt<<<1, 64, 0, g->stream>>>(g->clock, dt_seq ? g->dt_hist : nullptr, s);
+        SF_HIP(hipGetLastError());
+        if (int rc = sf_step_begin(g)) return rc;
+        for (int st = 0; st < nstages(g->desc.time); ++st)
+            if (int rc = sf_stage(g, st)) return rc;
+        if (int rc = sf_step_commit(g)) return rc;
+    }
+    g->fold_cfl = 0;
+    if (dt_seq && nsteps > 0) {
+        SF_HIP(hipMemcpyAsync(dt_seq, g->dt_hist, sizeof(double) * (size_t)nsteps,
+                              hipMemcpyDeviceToHost, g->stream));
+    }
+    return check_flags(g, "sf_run");
+}
+
+int sf_clock(sf_grid* g, double* time, long long* steps) {
+    if (!g) return fail(SF_E_ARG, "sf_clock: null grid");
+    if (int rc = set_device(g)) return rc;
+    Clock c;
+    SF_HIP(hipMemcpyAsync(&c, g->clock, sizeof c, hipMemcpyDeviceToHost, g->stream));
+    SF_HIP(hipStreamSynchronize(g->stream));
+    if (time) *time = c.time;
+    if (steps) *steps = c.steps;
+    return 0;
+}
+
+int sf_residual(sf_grid* g, double* soa_out) {
+    if (!g || !soa_out) return fail(SF_E_ARG, "sf_residual: null argument");
+    if (g->step_open) return fail(SF_E_STATE, "sf_residual: a step is open");
+    if (int rc = set_device(g)) return rc;
+    if (int rc = sf_step_prepare(g)) return rc;
+    const int scratch = (g->cur + 1) % 3;
+    SF_HIP(hipMemsetAsync(g->buf[scratch], 0, sizeof(double) * 5 * (size_t)g->G.P, g->stream));
+    StageArgs A{};
+    A.qout = g->buf[scratch];
+    A.flags = g->flags;
+    A.time = 3;  // residual only
+    A.dt = &g->clock->dt;
+    A.dtmin = &g->clock->dtmin;
+    if (int rc = run_sweeps(g, g->buf[g->cur], A)) return rc;
+    const long long N = (long long)g->G.n[0] * g->G.n[1] * g->G.n[2];
+    const unsigned nb = std::min<unsigned>(blocks_for(N, 256), 65535u);
+    k_pack<<<nb, 256, 0, g->stream>>>(g->G, g->buf[scratch], g->stage_io, 0);
+    SF_HIP(hipGetLastError());
+    SF_HIP(hipMemcpyAsync(soa_out, g->stage_io, sizeof(double) * 5 * (size_t)N,
+                          hipMemcpyDeviceToHost, g->stream));
+    SF_HIP(hipStreamSynchronize(g->stream));
+    return check_flags(g, "sf_residual");
+}
+
+int sf_error_flags(sf_grid* g, uint32_t* flags, int clear) {
+    if (!g || !flags) return fail(SF_E_ARG, "sf_error_flags: null argument");
+    if (int rc = set_device(g)) return rc;
+    unsigned h = 0;
+    SF_HIP(hipMemcpyAsync(&h, g->flags, sizeof h, hipMemcpyDeviceToHost, g->stream));
+    SF_HIP(hipStreamSynchronize(g->stream));
+    *flags = h;
+    if (clear) {
+        const unsigned zero = 0;
+        SF_HIP(hipMemcpy(g->flags, &zero, sizeof zero, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int sf_synchronize(sf_grid* g) {
+    if (!g) return fail(SF_E_ARG, "sf_synchronize: null grid");
+    if (int rc = set_device(g)) return rc;
+    SF_HIP(hipStreamSynchronize(g->stream));
+    return 0;
+}
+
+}  // extern "C"

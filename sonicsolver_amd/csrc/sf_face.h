@@ -1,0 +1,315 @@
+// sf_face.h -- per-face FP64 arithmetic of the explicit Euler hot path for
+// gfx950: characteristic WENO5-JS / TENO5 reconstruction on an axis-aligned
+// face followed by a Rusanov or HLLC flux.
+//
+// The operation order follows the reference functions cited at each
+// definition (paths relative to the sonicSolver source tree).  On an
+// axis-aligned face the unit normal and the tangent basis (l, m) are
+// signed unit vectors, so every product with one of their zero components
+// is dropped and every product with +-1 becomes a sign; for finite inputs
+// this changes nothing but the sign of a zero.  hipcc may contract a*b+c
+// into FMA, so results differ from the reference in the last bits only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sfamd {
+
+enum : unsigned {
+    kFlagState = 1u, kFlagChar = 2u, kFlagTeno = 4u, kFlagFlux = 8u, kFlagDt = 16u
+};
+
+// Tangent basis of Riemann::buildCharacteristicMatrix for n = e_AXIS
+// (SF_riemann.cpp:88-97):
+//   x: l = (-0, 1, 0), m = (0, 0, 1)
+//   y: l = (-1, 0, 0), m = (0, -0, 1)
+//   z: l = (-1, 0, 0), m = (0, -1, 0)
+template <int AXIS> struct Basis;
+template <> struct Basis<0> {
+    static constexpr int nd = 0, li = 1, mi = 2;
+    static constexpr double sl = 1.0, sm = 1.0;
+};
+template <> struct Basis<1> {
+    static constexpr int nd = 1, li = 0, mi = 2;
+    static constexpr double sl = -1.0, sm = 1.0;
+};
+template <> struct Basis<2> {
+    static constexpr int nd = 2, li = 0, mi = 1;
+    static constexpr double sl = -1.0, sm = -1.0;
+};
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
+// WENO5::weno5_core, src/methods/numerics/convection/SF_WENO5.cpp:42-62.
+__device__ __forceinline__ double weno5(double v0, double v1, double v2,
+                                        double v3, double v4) {
+    const double p0 = (2.0 * v0 - 7.0 * v1 + 11.0 * v2) / 6.0;
+    const double p1 = (-v1 + 5.0 * v2 + 2.0 * v3) / 6.0;
+    const double p2 = (2.0 * v2 + 5.0 * v3 - v4) / 6.0;
+    const double b0 = 13.0 / 12.0 * sq(v0 - 2.0 * v1 + v2)
+                    + 0.25 * sq(v0 - 4.0 * v1 + 3.0 * v2);
+    const double b1 = 13.0 / 12.0 * sq(v1 - 2.0 * v2 + v3)
+                    + 0.25 * sq(v1 - v3);
+    const double b2 = 13.0 / 12.0 * sq(v2 - 2.0 * v3 + v4)
+                    + 0.25 * sq(3.0 * v2 - 4.0 * v3 + v4);
+    const double eps = 1e-6;
+    const double a0 = 0.1 / sq(eps + b0);
+    const double a1 = 0.6 / sq(eps + b1);
+    const double a2 = 0.3 / sq(eps + b2);
+    return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+}
+
+// TENO5::teno5_core, src/methods/numerics/convection/SF_TENO5.cpp:44-92
+// (C = 1, q = 6, CT = 1e-5).  gamma^6 is formed by three products instead
+// of std::pow; the two agree to a few ulps, which can only matter where
+// gamma_k / sum lands within that distance of CT.
+__device__ __forceinline__ double teno5(double v0, double v1, double v2,
+                                        double v3, double v4, unsigned& flags) {
+    const double p0 = (2.0 * v0 - 7.0 * v1 + 11.0 * v2) / 6.0;
+    const double p1 = (-v1 + 5.0 * v2 + 2.0 * v3) / 6.0;
+    const double p2 = (2.0 * v2 + 5.0 * v3 - v4) / 6.0;
+    const double b0 = 13.0 / 12.0 * sq(v0 - 2.0 * v1 + v2)
+                    + 0.25 * sq(v0 - 4.0 * v1 + 3.0 * v2);
+    const double b1 = 13.0 / 12.0 * sq(v1 - 2.0 * v2 + v3)
+                    + 0.25 * sq(v1 - v3);
+    const double b2 = 13.0 / 12.0 * sq(v2 - 2.0 * v3 + v4)
+                    + 0.25 * sq(3.0 * v2 - 4.0 * v3 + v4);
+    const double tau5 = fabs(b0 - b2);
+    double g0 = 1.0 + tau5 / (b0 + 1.0e-40);
+    double g1 = 1.0 + tau5 / (b1 + 1.0e-40);
+    double g2 = 1.0 + tau5 / (b2 + 1.0e-40);
+    g0 = g0 * g0 * g0; g0 = g0 * g0;
+    g1 = g1 * g1 * g1; g1 = g1 * g1;
+    g2 = g2 * g2 * g2; g2 = g2 * g2;
+    const double gs = g0 + g1 + g2;
+    const double c0 = (g0 / gs < 1.0e-5) ? 0.0 : 1.0;
+    const double c1 = (g1 / gs < 1.0e-5) ? 0.0 : 1.0;
+    const double c2 = (g2 / gs < 1.0e-5) ? 0.0 : 1.0;
+    const double active = 0.1 * c0 + 0.6 * c1 + 0.3 * c2;
+    if (!(active > 0.0)) flags |= kFlagTeno;
+    return (0.1 * c0 * p0 + 0.6 * c1 * p1 + 0.3 * c2 * p2) / active;
+}
+
+template <int SCHEME>
+__device__ __forceinline__ double recon(double v0, double v1, double v2,
+                                        double v3, double v4, unsigned& flags) {
+    if constexpr (SCHEME == 1) return teno5(v0, v1, v2, v3, v4, flags);
+    else return weno5(v0, v1, v2, v3, v4);
+}
+
+// Closure of SingleFluidStateModel::close + PerfectGasEOS
+// (SF_singleFluidStateModel.cpp:49-71, SF_perfectGasEOS.h:24-32).
+struct Closed {
+    double u[3], p, c;
+};
+
+__device__ __forceinline__ Closed close_state(const double q[5], double gamma) {
+    Closed s;
+    const double rho = q[0];
+    s.u[0] = q[1] / rho;
+    s.u[1] = q[2] / rho;
+    s.u[2] = q[3] / rho;
+    const double kinetic = 0.5 * rho * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
+    const double e = (q[4] - kinetic) / rho;
+    s.p = (gamma - 1.0) * rho * e;
+    s.c = sqrt(gamma * s.p / rho);
+    return s;
+}
+
+// physicalFlux (SF_rusanovEOS.cpp:14-32) for n = e_ND.
+template <int ND>
+__device__ __forceinline__ void physical_flux(const double q[5], const Closed& s,
+                                              double f[5]) {
+    const double un = s.u[ND];
+    f[0] = q[0] * un;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[1 + d] = (d == ND) ? q[1 + d] * un + s.p : q[1 + d] * un;
+    f[4] = (q[4] + s.p) * un;
+}
+
+// RusanovEOS::faceFlux, SF_rusanovEOS.cpp:47-77.
+template <int ND>
+__device__ __forceinline__ void rusanov(const double qL[5], const double qR[5],
+                                        double gamma, double out[5]) {
+    const Closed l = close_state(qL, gamma), r = close_state(qR, gamma);
+    double fL[5], fR[5];
+    physical_flux<ND>(qL, l, fL);
+    physical_flux<ND>(qR, r, fR);
+    const double lambda = fmax(fabs(l.u[ND]) + l.c, fabs(r.u[ND]) + r.c);
+#pragma unroll
+    for (int v = 0; v < 5; ++v)
+        out[v] = 0.5 * (fL[v] + fR[v]) - 0.5 * lambda * (qR[v] - qL[v]);
+}
+
+// HLLC (no reference counterpart).  Toro, "Riemann Solvers and Numerical
+// Methods for Fluid Dynamics", 3rd ed.: Einfeldt wave speeds from Roe
+// averages, contact speed eq. (10.37), star states eq. (10.39) with the
+// tangential velocity of each side carried unchanged, flux eq. (10.38).
+template <int ND>
+__device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
+                                     double gamma, double out[5]) {
+    const Closed l = close_state(qL, gamma), r = close_state(qR, gamma);
+    const double unL = l.u[ND], unR = r.u[ND];
+    const double sL = sqrt(qL[0]), sR = sqrt(qR[0]);
+    const double inv = 1.0 / (sL + sR);
+    double ut[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ut[d] = (sL * l.u[d] + sR * r.u[d]) * inv;
+    const double HL = (qL[4] + l.p) / qL[0], HR = (qR[4] + r.p) / qR[0];
+    const double Ht = (sL * HL + sR * HR) * inv;
+    const double ct = sqrt((gamma - 1.0) * (Ht - 0.5 * (ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2])));
+    const double SL = fmin(unL - l.c, ut[ND] - ct);
+    const double SR = fmax(unR + r.c, ut[ND] + ct);
+    const double mL = qL[0] * (SL - unL), mR = qR[0] * (SR - unR);
+    const double Ss = (r.p - l.p + unL * mL - unR * mR) / (mL - mR);
+    if (0.0 <= SL) {
+        physical_flux<ND>(qL, l, out);
+        return;
+    }
+    if (SR <= 0.0) {
+        physical_flux<ND>(qR, r, out);
+        return;
+    }
+    const bool left = 0.0 <= Ss;
+    double q[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) q[v] = left ? qL[v] : qR[v];
+    const Closed& s = left ? l : r;
+    const double un = left ? unL : unR;
+    const double SK = left ? SL : SR;
+    const double mK = left ? mL : mR;
+    double f[5];
+    physical_flux<ND>(q, s, f);
+    const double scale = mK / (SK - Ss);
+    double us[5];
+    us[0] = scale;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        us[1 + d] = scale * ((d == ND) ? s.u[d] + (Ss - un) : s.u[d]);
+    us[4] = scale * (q[4] / q[0] + (Ss - un) * (Ss + s.p / mK));
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = f[v] + SK * (us[v] - q[v]);
+}
+
+// Flux::characteristicWENOFaceFlux<6> (SF_WENO.h:185-245) on a face of
+// axis AXIS whose cofactor is area * e_AXIS.  st[s][v] holds the stencil
+// nodes at offsets -2..+3 around the face.  Returns the area-weighted flux.
+template <int AXIS, int SCHEME, int FLUX>
+__device__ __forceinline__ void face_flux(const double st[6][5], double gamma,
+                                          double area, double out[5],
+                                          unsigned& flags) {
+    using B = Basis<AXIS>;
+    constexpr int nd = B::nd, li = B::li, mi = B::mi;
+    constexpr double sl = B::sl, sm = B::sm;
+
+    // Face average (SF_WENO.h:207-211) and the eigenvector matrices of
+    // Riemann::buildCharacteristicMatrix (SF_riemann.cpp:69-123).
+    double qa[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qa[v] = 0.5 * (st[2][v] + st[3][v]);
+    const double invRho0 = 1.0 / qa[0];
+    const double ke = 0.5 * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * invRho0;
+    const double p = (gamma - 1.0) * (qa[4] - ke);
+    if (!(qa[0] > 0.0) || !(p > 0.0)) flags |= kFlagChar;
+    const double invR = 1.0 / qa[0];
+    double u[3];
+    u[0] = qa[1] * invR;
+    u[1] = qa[2] * invR;
+    u[2] = qa[3] * invR;
+    const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double c = sqrt(gamma * p * invR);
+    const double H = (qa[4] + p) * invR;
+    const double un = u[nd];
+    const double ul = sl * u[li];
+    const double um = sm * u[mi];
+    const double b1 = (gamma - 1.0) / (c * c);
+    const double b2 = 0.5 * b1;
+    const double x1 = 1.0 / (2.0 * c);
+
+    // rows 0, 3, 4 of L (rows 1, 2 are sparse on an axis-aligned face)
+    const double L00 = 1.0 - b2 * q2;
+    const double L0[3] = {b1 * u[0], b1 * u[1], b1 * u[2]};
+    const double L04 = -b1;
+    const double A3 = 0.5 * b2 * q2 + x1 * un;
+    const double A4 = 0.5 * b2 * q2 - x1 * un;
+    double C3[3], C4[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        C3[d] = (d == nd) ? -x1 - b2 * u[d] : -(b2 * u[d]);
+        C4[d] = (d == nd) ? x1 - b2 * u[d] : -(b2 * u[d]);
+    }
+
+    // Characteristic projections (gemv5 with L, SF_riemann.h:33-37) and
+    // the scalar reconstructions: left stencil s = 0..4, right stencil
+    // mirrored s = 5..1 (fillScalarWENOStencils<6>, SF_WENO.h:79-98).
+    double wL[5], wR[5];
+    {
+        double ch[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+            ch[s] = L00 * st[s][0] + L0[0] * st[s][1] + L0[1] * st[s][2]
+                  + L0[2] * st[s][3] + L04 * st[s][4];
+        wL[0] = recon<SCHEME>(ch[0], ch[1], ch[2], ch[3], ch[4], flags);
+        wR[0] = recon<SCHEME>(ch[5], ch[4], ch[3], ch[2], ch[1], flags);
+    }
+    {
+        double ch[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) ch[s] = -ul * st[s][0] + sl * st[s][1 + li];
+        wL[1] = recon<SCHEME>(ch[0], ch[1], ch[2], ch[3], ch[4], flags);
+        wR[1] = recon<SCHEME>(ch[5], ch[4], ch[3], ch[2], ch[1], flags);
+    }
+    {
+        double ch[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) ch[s] = -um * st[s][0] + sm * st[s][1 + mi];
+        wL[2] = recon<SCHEME>(ch[0], ch[1], ch[2], ch[3], ch[4], flags);
+        wR[2] = recon<SCHEME>(ch[5], ch[4], ch[3], ch[2], ch[1], flags);
+    }
+    {
+        double ch[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+            ch[s] = A3 * st[s][0] + C3[0] * st[s][1] + C3[1] * st[s][2]
+                  + C3[2] * st[s][3] + b2 * st[s][4];
+        wL[3] = recon<SCHEME>(ch[0], ch[1], ch[2], ch[3], ch[4], flags);
+        wR[3] = recon<SCHEME>(ch[5], ch[4], ch[3], ch[2], ch[1], flags);
+    }
+    {
+        double ch[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+            ch[s] = A4 * st[s][0] + C4[0] * st[s][1] + C4[1] * st[s][2]
+                  + C4[2] * st[s][3] + b2 * st[s][4];
+        wL[4] = recon<SCHEME>(ch[0], ch[1], ch[2], ch[3], ch[4], flags);
+        wR[4] = recon<SCHEME>(ch[5], ch[4], ch[3], ch[2], ch[1], flags);
+    }
+
+    // Back to conservative variables with R (gemv5).
+    double qL[5], qR[5];
+    const double Hm = H - c * un, Hp = H + c * un;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const double* w = side ? wR : wL;
+        double* q = side ? qR : qL;
+        q[0] = w[0] + w[3] + w[4];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (d == nd)
+                q[1 + d] = u[d] * w[0] + (u[d] - c) * w[3] + (u[d] + c) * w[4];
+            else if (d == li)
+                q[1 + d] = u[d] * w[0] + sl * w[1] + u[d] * w[3] + u[d] * w[4];
+            else
+                q[1 + d] = u[d] * w[0] + sm * w[2] + u[d] * w[3] + u[d] * w[4];
+        }
+        q[4] = 0.5 * q2 * w[0] + ul * w[1] + um * w[2] + Hm * w[3] + Hp * w[4];
+    }
+
+    double f[5];
+    if constexpr (FLUX == 1) hllc<nd>(qL, qR, gamma, f);
+    else rusanov<nd>(qL, qR, gamma, f);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = f[v] * area;
+}
+
+}  // namespace sfamd

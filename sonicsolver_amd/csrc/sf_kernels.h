@@ -1,0 +1,436 @@
+// sf_kernels.h -- gfx950 kernels of the explicit Euler hot path (included by sf_capi.hip).
+//
+// Data layout in HBM: conserved state as structure-of-arrays FP64,
+// q[v*P + (k*m1 + j)*m0 + i], with a 3-deep ghost layer on every active
+// axis (m_d = n_d + 6) and none on an inactive one.  Work per RK stage:
+//
+//   ghost fill (O(surface))      k_bc_copy4 / k_bc_energy / k_bc_periodic
+//   x sweep                      k_sweep_x:     one lane per x face, the
+//                                               flux difference via LDS
+//   y / z sweeps                 k_sweep_march: one lane per (x, column
+//                                               segment), marching along the
+//                                               axis with the 6-node stencil
+//                                               in registers, so each node is
+//                                               loaded once and each face
+//                                               evaluated once
+//   the last active sweep also applies the stage update, validates the new
+//   state and (on the last stage) folds the CFL scan of the next step into
+//   a wave-level minimum plus one 64-bit atomic per wave.
+//
+// Sweeps accumulate S = sum_d (F_d(i) - F_d(i-1)) in the order x, y, z and
+// the update multiplies by Jac once, as Math::localSpatialResidual does
+// (SF_residualAssembly.h:12-32).
+#pragma once
+
+#include "sf_face.h"
+
+namespace sfamd {
+
+struct GridDev {
+    int n[3];
+    int g[3];
+    int m[3];
+    long long P;        // padded nodes per variable
+    long long stride[3];
+    int active[3];
+    int bc[3];          // 0 transmissive, 1 periodic
+    double jac, area[3], metric[3], gamma;
+};
+
+enum { kCopyS = 0, kAccumS = 1, kFinal = 2 };
+
+// Stage update parameters (Time::Explicit, SF_explicit.cpp:193-238).
+struct StageArgs {
+    const double* q0;        // step-start state (SSPRK3, RK4)
+    double* qout;            // new state
+    double* k[3];            // RK4 stage residuals k1..k3
+    const double* dt;        // device scalar: dt of this step
+    unsigned long long* dtmin;  // CFL accumulator (bits of a positive double)
+    unsigned* flags;
+    double base, incr;       // SSPRK3 base / increment weights
+    double cfl;
+    int time;                // 0 Euler, 1 SSPRK3, 2 RK4
+    int stage;
+    int reduce_dt;           // fold the next step's CFL scan into this update
+};
+
+__device__ __forceinline__ long long node_index(const GridDev& G, int i, int j, int k) {
+    return (long long)(k + G.g[2]) * G.stride[2] + (long long)(j + G.g[1]) * G.stride[1]
+         + (i + G.g[0]);
+}
+
+__device__ __forceinline__ bool solved_node(const GridDev& G, int i, int j, int k) {
+    const int p[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (G.active[d] && G.bc[d] == 0 && (p[d] == 0 || p[d] == G.n[d] - 1)) return false;
+    return true;
+}
+
+// Local CFL step of RusanovEOS::deltaT (SF_rusanovEOS.cpp:110-147).
+__device__ __forceinline__ double local_dt(const GridDev& G, const double q[5], double cfl,
+                                           unsigned& flags) {
+    const Closed s = close_state(q, G.gamma);
+    double spectral = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (G.active[d]) spectral += fabs(s.u[d] * G.metric[d]) + s.c * G.metric[d];
+    if (!(spectral > 0.0) || !isfinite(spectral)) flags |= kFlagDt;
+    return cfl / spectral;
+}
+
+__device__ __forceinline__ void wave_min_atomic(double v, unsigned long long* target) {
+    // positive doubles order like their bit patterns
+    unsigned long long b = (v > 0.0 && isfinite(v)) ? (unsigned long long)__double_as_longlong(v)
+                                                    : 0x7FF0000000000000ull;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = o < b ? o : b;
+    }
+    if ((threadIdx.x & 63) == 0 && b != 0x7FF0000000000000ull) atomicMin(target, b);
+}
+
+__device__ __forceinline__ void raise_flags(unsigned f, unsigned* flags) {
+    if (f) atomicOr(flags, f);
+}
+
+// The reference updates the state in place, so a node that the stage does
+// not advance (a solver-boundary node) keeps the value of the last ghost
+// fill; with separate input and output buffers it is copied across.
+__device__ __forceinline__ void carry_node(const GridDev& G, const StageArgs& A, long long id,
+                                           const double* __restrict__ q) {
+    if (A.time == 3) return;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) A.qout[v * G.P + id] = q[v * G.P + id];
+}
+
+// Stage update of one node; S is the accumulated flux difference.
+__device__ __forceinline__ void stage_update(const GridDev& G, const StageArgs& A, long long id,
+                                             const double* __restrict__ q, const double S[5],
+                                             unsigned& flags, double& dt_local) {
+    if (A.time == 3) {  // residual only (diagnostics)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) A.qout[v * G.P + id] = G.jac * S[v];
+        return;
+    }
+    const double dt = *A.dt;
+    double qn[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const double R = G.jac * S[v];
+        const double qc = q[v * G.P + id];
+        if (A.time == 1) {
+            const double euler = qc - dt * R;
+            qn[v] = (A.stage == 0 ? 0.0 : A.base * A.q0[v * G.P + id]) + A.incr * euler;
+        } else if (A.time == 0) {
+            qn[v] = qc - dt * R;
+        } else if (A.stage < 3) {
+            A.k[A.stage][v * G.P + id] = R;
+            qn[v] = A.q0[v * G.P + id] - A.incr * dt * R;
+        } else {
+            qn[v] = A.q0[v * G.P + id]
+                  - dt / 6.0 * (A.k[0][v * G.P + id] + 2.0 * A.k[1][v * G.P + id]
+                                + 2.0 * A.k[2][v * G.P + id] + R);
+        }
+        A.qout[v * G.P + id] = qn[v];
+    }
+    // validateEquationState (SF_equationState.h:25-57): rho, p > 0, finite
+    const double ke = 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) / qn[0];
+    const double eint = qn[4] - ke;
+    if (!(qn[0] > 0.0) || !(eint > 0.0) || !isfinite(eint)) flags |= kFlagState;
+    if (A.reduce_dt) dt_local = local_dt(G, qn, A.cfl, flags);
+}
+
+// ---------------------------------------------------------------------------
+// x sweep: lane t of a 128-lane block evaluates face (i0-1+t)+1/2; lanes
+// 1..127 then form the difference for node i0-1+t.  Blocks cover 127 nodes.
+// ---------------------------------------------------------------------------
+constexpr int kXBlock = 128;
+
+template <int SCHEME, int FLUX, int MODE>
+__global__ __launch_bounds__(kXBlock) void k_sweep_x(GridDev G, const double* __restrict__ q,
+                                                     double* __restrict__ S, StageArgs A) {
+    __shared__ double sh[5][kXBlock];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * (kXBlock - 1);
+    const int j = blockIdx.y, k = blockIdx.z;
+    const int a = i0 - 1 + t;  // lower node of this lane's face
+    unsigned flags = 0;
+    if (a <= G.n[0] - 1) {
+        const long long base = node_index(G, a, j, k);
+        double st[6][5];
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + base + (s - 2)];
+        double f[5];
+        face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f, flags);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            if (!isfinite(f[v])) flags |= kFlagFlux;
+            sh[v][t] = f[v];
+        }
+    }
+    __syncthreads();
+    double dt_local = INFINITY;
+    if (t >= 1 && a <= G.n[0] - 1) {
+        const long long id = node_index(G, a, j, k);
+        double s5[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            const double d = sh[v][t] - sh[v][t - 1];
+            s5[v] = (MODE == kCopyS) ? d : S[v * G.P + id] + d;
+        }
+        if (MODE == kFinal) {
+            if (solved_node(G, a, j, k)) stage_update(G, A, id, q, s5, flags, dt_local);
+            else carry_node(G, A, id, q);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
+        }
+    }
+    raise_flags(flags, A.flags);
+    if (MODE == kFinal && A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);
+}
+
+// ---------------------------------------------------------------------------
+// y / z sweep: lane = x index, blockIdx.y = the remaining transverse index,
+// blockIdx.z = segment of SEG nodes along the sweep axis.
+// ---------------------------------------------------------------------------
+constexpr int kMarchBlock = 64;
+
+template <int AXIS, int SCHEME, int FLUX, int MODE>
+__global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
+                                                             const double* __restrict__ q,
+                                                             double* __restrict__ S, StageArgs A,
+                                                             int seg) {
+    const int i = blockIdx.x * kMarchBlock + threadIdx.x;
+    const int o = blockIdx.y;
+    const int a0 = blockIdx.z * seg;
+    const int a1 = min(a0 + seg, G.n[AXIS]);
+    unsigned flags = 0;
+    double dt_local = INFINITY;
+    if (i < G.n[0]) {
+        const int j0 = (AXIS == 1) ? a0 : o;
+        const int k0 = (AXIS == 2) ? a0 : o;
+        const long long step = G.stride[AXIS];
+        // node (a0 - 3) along the axis
+        long long id = node_index(G, i, j0, k0) - 3 * step;
+        double st[6][5];
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + id + s * step];
+        double fprev[5];
+        face_flux<AXIS, SCHEME, FLUX>(st, G.gamma, G.area[AXIS], fprev, flags);
+        id += 3 * step;  // node a0
+        for (int a = a0; a < a1; ++a, id += step) {
+#pragma unroll
+            for (int s = 0; s < 5; ++s)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) st[s][v] = st[s + 1][v];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) st[5][v] = q[v * G.P + id + 3 * step];
+            double f[5];
+            face_flux<AXIS, SCHEME, FLUX>(st, G.gamma, G.area[AXIS], f, flags);
+            double s5[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                if (!isfinite(f[v])) flags |= kFlagFlux;
+                const double d = f[v] - fprev[v];
+                s5[v] = (MODE == kCopyS) ? d : S[v * G.P + id] + d;
+                fprev[v] = f[v];
+            }
+            if (MODE == kFinal) {
+                const int jj = (AXIS == 1) ? a : o;
+                const int kk = (AXIS == 2) ? a : o;
+                double dl = INFINITY;
+                if (solved_node(G, i, jj, kk)) stage_update(G, A, id, q, s5, flags, dl);
+                else carry_node(G, A, id, q);
+                dt_local = fmin(dt_local, dl);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
+            }
+        }
+    }
+    raise_flags(flags, A.flags);
+    if (MODE == kFinal && A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);
+}
+
+// ---------------------------------------------------------------------------
+// Standalone CFL scan (first step, or sf_compute_dt).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cfl(GridDev G, const double* __restrict__ q, double cfl,
+                                             unsigned long long* dtmin, unsigned* flags_out) {
+    const long long N = (long long)G.n[0] * G.n[1] * G.n[2];
+    unsigned flags = 0;
+    double best = INFINITY;
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < N;
+         u += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(u % G.n[0]);
+        const int j = (int)((u / G.n[0]) % G.n[1]);
+        const int k = (int)(u / ((long long)G.n[0] * G.n[1]));
+        if (!solved_node(G, i, j, k)) continue;
+        const long long id = node_index(G, i, j, k);
+        double qq[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) qq[v] = q[v * G.P + id];
+        best = fmin(best, local_dt(G, qq, cfl, flags));
+    }
+    raise_flags(flags, flags_out);
+    wave_min_atomic(best, dtmin);
+}
+
+// dt = min(scan, max_dt, limit); reset the accumulator; advance the record.
+struct Clock {
+    double dt;
+    double time;
+    long long steps;
+    unsigned long long dtmin;
+    double max_dt;
+    double limit;
+};
+
+__global__ void k_finalize_dt(Clock* c, double* dt_hist, int slot) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double dt = __longlong_as_double((long long)c->dtmin);
+    dt = fmin(dt, c->max_dt);
+    dt = fmin(dt, c->limit);
+    c->dt = dt;
+    c->dtmin = 0x7FF0000000000000ull;
+    if (dt_hist) dt_hist[slot] = dt;
+}
+
+__global__ void k_set_dt(Clock* c, double dt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->dt = dt;
+}
+
+__global__ void k_set_limits(Clock* c, double max_dt, double limit, int reset_scan) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->max_dt = max_dt;
+        c->limit = limit;
+        if (reset_scan) c->dtmin = 0x7FF0000000000000ull;
+    }
+}
+
+__global__ void k_commit_clock(Clock* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->time += c->dt;
+        c->steps += 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ghost fill.  Thread per boundary-plane point (2 sides x plane).
+// ---------------------------------------------------------------------------
+template <int AXIS>
+__device__ __forceinline__ bool plane_point(const GridDev& G, long long t, int p[3], int& side) {
+    constexpr int e1 = (AXIS + 1) % 3, e2 = (AXIS + 2) % 3;
+    const long long plane = (long long)G.n[e1] * G.n[e2];
+    if (t >= 2 * plane) return false;
+    side = (int)(t / plane);
+    const long long r = t % plane;
+    p[e1] = (int)(r % G.n[e1]);
+    p[e2] = (int)(r / G.n[e1]);
+    p[AXIS] = side ? G.n[AXIS] - 1 : 0;
+    return true;
+}
+
+// zeroGradient density + momentum (SF_zeroGradient.cpp:88-130).
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_bc_copy4(GridDev G, double* __restrict__ q) {
+    int p[3], side;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (!plane_point<AXIS>(G, t, p, side)) return;
+    const long long step = G.stride[AXIS];
+    const long long b = node_index(G, p[0], p[1], p[2]);
+    const long long s = side ? b - step : b + step;
+    const long long dir = side ? step : -step;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double val = q[v * G.P + s];
+        q[v * G.P + b] = val;
+#pragma unroll
+        for (int l = 1; l <= 3; ++l) q[v * G.P + b + l * dir] = val;
+    }
+}
+
+__device__ __forceinline__ double pressure_of(const double* q, long long P, long long id,
+                                              double gamma) {
+    double qq[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qq[v] = q[v * P + id];
+    return close_state(qq, gamma).p;
+}
+
+// totalEnergyFromPressure (SF_singleFluidStateModel.cpp:73-109).
+__device__ __forceinline__ double energy_from_pressure(const double* q, long long P, long long id,
+                                                       double gamma, double p) {
+    const double r = q[id], m1 = q[P + id], m2 = q[2 * P + id], m3 = q[3 * P + id];
+    const double e = 0.0 + p / ((gamma - 1.0) * r);
+    const double kinetic = 0.5 * (1.0 / r) * (m1 * m1 + m2 * m2 + m3 * m3);
+    return r * e + kinetic;
+}
+
+// zeroGradient pressure -> energy (SF_thermodynamicClosure.cpp:219-281).
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_bc_energy(GridDev G, double* __restrict__ q,
+                                                   unsigned* flags_out) {
+    int p[3], side;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (!plane_point<AXIS>(G, t, p, side)) return;
+    const long long step = G.stride[AXIS];
+    const long long b = node_index(G, p[0], p[1], p[2]);
+    const long long s = side ? b - step : b + step;
+    const long long dir = side ? step : -step;
+    const double ps = pressure_of(q, G.P, s, G.gamma);
+    q[4 * G.P + b] = energy_from_pressure(q, G.P, b, G.gamma, ps);
+    const double pb = pressure_of(q, G.P, b, G.gamma);
+    if (!(ps > 0.0) || !(pb > 0.0)) atomicOr(flags_out, (unsigned)kFlagState);
+#pragma unroll
+    for (int l = 1; l <= 3; ++l)
+        q[4 * G.P + b + l * dir] = energy_from_pressure(q, G.P, b + l * dir, G.gamma, pb);
+}
+
+// Periodic: ghost layer l on either side takes the node (index mod n).
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_bc_periodic(GridDev G, double* __restrict__ q) {
+    int p[3], side;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (!plane_point<AXIS>(G, t, p, side)) return;
+    const long long step = G.stride[AXIS];
+    const int n = G.n[AXIS];
+    p[AXIS] = 0;
+    const long long base = node_index(G, p[0], p[1], p[2]);  // node 0 on the axis
+#pragma unroll
+    for (int l = 1; l <= 3; ++l) {
+        // low side: ghost -l <- node n-l ; high side: ghost n-1+l <- node l-1
+        const long long dst = side ? base + (long long)(n - 1 + l) * step : base - l * step;
+        const long long src = side ? base + (long long)((l - 1) % n) * step
+                                   : base + (long long)(((n - l) % n + n) % n) * step;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) q[v * G.P + dst] = q[v * G.P + src];
+    }
+}
+
+// Copy between the caller's unique-node SoA and the padded layout.
+__global__ __launch_bounds__(256) void k_pack(GridDev G, const double* __restrict__ src,
+                                              double* __restrict__ dst, int to_padded) {
+    const long long N = (long long)G.n[0] * G.n[1] * G.n[2];
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < N;
+         u += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(u % G.n[0]);
+        const int j = (int)((u / G.n[0]) % G.n[1]);
+        const int k = (int)(u / ((long long)G.n[0] * G.n[1]));
+        const long long id = node_index(G, i, j, k);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            if (to_padded) dst[v * G.P + id] = src[v * N + u];
+            else dst[v * N + u] = src[v * G.P + id];
+        }
+    }
+}
+
+}  // namespace sfamd

@@ -1,0 +1,127 @@
+"""Synthetic initial states of the benchmark configurations.
+
+Each generator returns ``(desc_kwargs, q)`` where ``q`` is the conserved
+state, variable-major over the unique nodes, ``q[v, (k*ny + j)*nx + i]``,
+v = (rho, rhoU, rhoV, rhoW, rhoE).  Grids are vertex-centred as in the
+reference (solution points are mesh nodes).  Random draws use splitmix64
+so the inputs are reproducible bit for bit on any host.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+GAMMA = 1.4
+_MASK = (1 << 64) - 1
+
+
+def splitmix64(seed: int, count: int) -> np.ndarray:
+    """``count`` successive outputs of splitmix64 (Steele, Lea & Flood 2014)."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed & _MASK) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, count: int) -> np.ndarray:
+    """Uniform doubles in [0, 1) from the top 53 bits of splitmix64."""
+    return (splitmix64(seed, count) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def conserved(rho, u, v, w, p, gamma=GAMMA) -> np.ndarray:
+    rho = np.asarray(rho, dtype=np.float64)
+    u, v, w, p = (np.broadcast_to(np.asarray(a, dtype=np.float64), rho.shape) for a in (u, v, w, p))
+    e = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    return np.stack([rho, rho * u, rho * v, rho * w, e]).reshape(5, -1).copy()
+
+
+def primitive(q, gamma=GAMMA):
+    q = np.asarray(q).reshape(5, -1)
+    rho = q[0]
+    u, v, w = q[1] / rho, q[2] / rho, q[3] / rho
+    p = (gamma - 1.0) * (q[4] - 0.5 * rho * (u * u + v * v + w * w))
+    return rho, u, v, w, p
+
+
+def sod_1d(cells: int = 1000, scheme="weno5", flux="rusanov", time="ssprk3"):
+    """1D shock tube on x in [0, 1]: cells+1 nodes, (1, 0, 1) on the nodes
+    left of the midpoint (the mesh generator's ``leftHalf`` set holds nodes
+    i < nx/2), (0.125, 0, 0.1) on the others."""
+    nx = cells + 1
+    i = np.arange(nx)
+    left = i < nx // 2
+    rho = np.where(left, 1.0, 0.125)
+    p = np.where(left, 1.0, 0.1)
+    q = conserved(rho, 0.0, 0.0, 0.0, p)
+    desc = dict(n=(nx, 1, 1), active=(1, 0, 0), bc=("transmissive",) * 3,
+                d=(1.0 / cells, 1.0, 1.0), scheme=scheme, flux=flux, time=time)
+    return desc, q
+
+
+def vortex_state(x, y, t, beta=5.0, length=10.0, u0=1.0, v0=1.0, gamma=GAMMA):
+    """Exact isentropic vortex (Shu 1998) of strength beta advected by (u0, v0),
+    on the periodic square [0, length)^2, at time t."""
+    xc = (length / 2 + u0 * t) % length
+    yc = (length / 2 + v0 * t) % length
+    dx = (x - xc + length / 2) % length - length / 2
+    dy = (y - yc + length / 2) % length - length / 2
+    r2 = dx * dx + dy * dy
+    f = beta / (2.0 * math.pi) * np.exp(0.5 * (1.0 - r2))
+    u = u0 - f * dy
+    v = v0 + f * dx
+    T = 1.0 - (gamma - 1.0) * beta * beta / (8.0 * gamma * math.pi ** 2) * np.exp(1.0 - r2)
+    rho = T ** (1.0 / (gamma - 1.0))
+    p = rho ** gamma
+    return rho, u, v, p
+
+
+def vortex_2d(n: int = 2048, noise: float = 1e-3, seed: int = 2026, scheme="teno5",
+              flux="hllc", time="ssprk3", length: float = 10.0):
+    """Isentropic vortex on the periodic square [0, 10)^2 with n x n unique
+    nodes, plus rho *= 1 + noise*U[-1, 1] (pressure unchanged)."""
+    h = length / n
+    y, x = np.meshgrid(np.arange(n) * h, np.arange(n) * h, indexing="ij")
+    rho, u, v, p = vortex_state(x, y, 0.0, length=length)
+    if noise:
+        rho = rho * (1.0 + noise * (2.0 * uniform01(seed, n * n).reshape(n, n) - 1.0))
+    q = conserved(rho, u, v, 0.0, p)
+    desc = dict(n=(n, n, 1), active=(1, 1, 0), bc=("periodic", "periodic", "transmissive"),
+                d=(h, h, 1.0), scheme=scheme, flux=flux, time=time)
+    return desc, q
+
+
+def tgv_3d(n: int = 512, seed: int = 2026, scheme="weno5", flux="hllc", time="ssprk3"):
+    """Taylor-Green-type field on the periodic cube [0, 2 pi)^3:
+    u = sin(x+p1)cos(y+p2)cos(z+p3), v = -cos(x+p1)sin(y+p2)cos(z+p3), w = 0,
+    rho = 1, p = 100 + (rho/16)(cos 2x + cos 2y)(cos 2z + 2); phases p_i
+    uniform in [0, 2 pi) from splitmix64(seed)."""
+    h = 2.0 * math.pi / n
+    ph = 2.0 * math.pi * uniform01(seed, 3)
+    c = np.arange(n) * h
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    u = np.sin(x + ph[0]) * np.cos(y + ph[1]) * np.cos(z + ph[2])
+    v = -np.cos(x + ph[0]) * np.sin(y + ph[1]) * np.cos(z + ph[2])
+    rho = np.ones_like(x)
+    p = 100.0 + rho / 16.0 * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
+    q = conserved(rho, u, v, 0.0, p)
+    desc = dict(n=(n, n, n), active=(1, 1, 1), bc=("periodic",) * 3, d=(h, h, h),
+                scheme=scheme, flux=flux, time=time)
+    return desc, q
+
+
+def smooth_field(n, seed: int = 7, amp: float = 0.2):
+    """A smooth positive state with velocity in every active direction (tests)."""
+    nx, ny, nz = (list(n) + [1, 1])[:3]
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r = uniform01(seed, 6)
+    tx, ty, tz = 2 * np.pi * x / nx, 2 * np.pi * y / max(ny, 1), 2 * np.pi * z / max(nz, 1)
+    rho = 1.0 + amp * np.sin(tx + 6 * r[0]) * np.cos(ty + 6 * r[1]) + 0.5 * amp * np.sin(tz + 6 * r[2])
+    u = 0.5 + amp * np.cos(ty + 6 * r[3])
+    v = -0.3 + amp * np.sin(tx + 6 * r[4])
+    w = 0.2 * np.cos(tz + 6 * r[5])
+    p = 1.0 + amp * np.cos(tx + ty)
+    return conserved(rho, u, v, w, p)
