@@ -156,6 +156,22 @@ int sf_synchronize(sf_grid *grid);
 /* The HIP stream (hipStream_t) the grid queues its work on. */
 int sf_grid_stream(sf_grid *grid, void **stream);
 
+/* ---- kernel timing -----------------------------------------------------------
+ * No reference counterpart (the reference has no timers).  When enabled,
+ * every launch of the kernels below is bracketed by HIP events recorded on
+ * the grid's stream; sf_profile_read waits for them and returns the summed
+ * duration and the launch count.  Enabling resets the counters. */
+enum {
+    SF_KERNEL_SWEEP_X = 0,   /* x-face flux sweep                               */
+    SF_KERNEL_SWEEP_Y = 1,   /* y-face flux sweep                               */
+    SF_KERNEL_SWEEP_Z = 2,   /* z-face flux sweep                               */
+    SF_KERNEL_GHOST = 3,     /* one ghost fill (all its boundary kernels)       */
+    SF_KERNEL_CFL = 4,       /* standalone CFL scan                             */
+    SF_KERNEL_COUNT = 5
+};
+int sf_profile_enable(sf_grid *grid, int enable);
+int sf_profile_read(sf_grid *grid, int kernel, double *total_ms, long long *launches);
+
 #ifdef __cplusplus
 }
 #endif

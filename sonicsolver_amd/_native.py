@@ -21,6 +21,8 @@ BCS = {"transmissive": 0, "zerogradient": 0, "periodic": 1}
 
 FLAG_NAMES = {1: "state", 2: "face-average", 4: "teno-cutoff", 8: "flux", 16: "dt"}
 
+KERNELS = {"sweep_x": 0, "sweep_y": 1, "sweep_z": 2, "ghost": 3, "cfl": 4}
+
 
 class GridDesc(ctypes.Structure):
     _fields_ = [
@@ -74,6 +76,8 @@ def _declare(lib):
         "sf_residual": [_VP, _P],
         "sf_error_flags": [_VP, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int],
         "sf_synchronize": [_VP],
+        "sf_profile_enable": [_VP, ctypes.c_int],
+        "sf_profile_read": [_VP, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_longlong)],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -214,6 +218,15 @@ class Grid:
         f = ctypes.c_uint32(0)
         _check(load().sf_error_flags(self._h, ctypes.byref(f), int(clear)))
         return f.value
+
+    def profile(self, enable: bool = True):
+        _check(load().sf_profile_enable(self._h, int(enable)))
+
+    def profile_read(self, kernel: int):
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_longlong(0)
+        _check(load().sf_profile_read(self._h, int(kernel), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def synchronize(self):
         _check(load().sf_synchronize(self._h))

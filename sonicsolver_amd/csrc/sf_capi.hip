@@ -12,6 +12,8 @@
 #include <limits>
 #include <new>
 #include <string>
+#include <utility>
+#include <vector>
 
 using namespace sfamd;
 
@@ -60,6 +62,10 @@ struct sf_grid {
     int fold_cfl = 0;
     double max_dt = std::numeric_limits<double>::max();
     double limit = std::numeric_limits<double>::max();
+    // kernel timing (sf_profile_*): event pairs recorded around launches
+    int profiling = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, int>> ev_used;  // (kernel, index of start event)
 };
 
 namespace {
@@ -101,11 +107,32 @@ int check_desc(const sf_grid_desc* d) {
 
 unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Start / stop of a timed section on the grid's stream (no-ops unless
+// profiling is enabled).
+int prof_start(sf_grid* g, int kernel) {
+    if (!g->profiling) return -1;
+    const size_t need = 2 * (g->ev_used.size() + 1);
+    while (g->ev_pool.size() < need) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        g->ev_pool.push_back(e);
+    }
+    const int idx = (int)(2 * g->ev_used.size());
+    g->ev_used.push_back({kernel, idx});
+    hipEventRecord(g->ev_pool[idx], g->stream);
+    return idx;
+}
+
+void prof_stop(sf_grid* g, int idx) {
+    if (idx >= 0) hipEventRecord(g->ev_pool[idx + 1], g->stream);
+}
+
 // Ghost fill in the order of Boundary::Applicator::apply (density and
 // momentum for every transmissive set, then energy from pressure, axis by
 // axis), then the periodic axes.
 int fill_ghosts(sf_grid* g, double* q) {
     const GridDev& G = g->G;
+    const int ev = prof_start(g, SF_KERNEL_GHOST);
     auto plane = [&](int a) {
         return 2LL * G.n[(a + 1) % 3] * G.n[(a + 2) % 3];
     };
@@ -130,6 +157,7 @@ int fill_ghosts(sf_grid* g, double* q) {
         if (a == 1) k_bc_periodic<1><<<nb, 256, 0, g->stream>>>(G, q);
         if (a == 2) k_bc_periodic<2><<<nb, 256, 0, g->stream>>>(G, q);
     }
+    prof_stop(g, ev);
     SF_HIP(hipGetLastError());
     return 0;
 }
@@ -137,9 +165,11 @@ int fill_ghosts(sf_grid* g, double* q) {
 template <int SCHEME, int FLUX, int MODE>
 void launch_axis(sf_grid* g, int axis, const double* q, const StageArgs& A) {
     const GridDev& G = g->G;
+    const int ev = prof_start(g, SF_KERNEL_SWEEP_X + axis);
     if (axis == 0) {
         dim3 grid(blocks_for(G.n[0], kXBlock - 1), G.n[1], G.n[2]);
         k_sweep_x<SCHEME, FLUX, MODE><<<grid, kXBlock, 0, g->stream>>>(G, q, g->S, A);
+        prof_stop(g, ev);
         return;
     }
     const int other = axis == 1 ? G.n[2] : G.n[1];
@@ -150,6 +180,7 @@ void launch_axis(sf_grid* g, int axis, const double* q, const StageArgs& A) {
     else
         k_sweep_march<2, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, g->stream>>>(G, q, g->S, A,
                                                                                  kMarchSeg);
+    prof_stop(g, ev);
 }
 
 template <int SCHEME, int FLUX>
@@ -210,7 +241,9 @@ int scan_dt(sf_grid* g, double cfl) {
         k_set_limits<<<1, 64, 0, g->stream>>>(g->clock, g->max_dt, g->limit, 1);
         const long long N = (long long)g->G.n[0] * g->G.n[1] * g->G.n[2];
         const unsigned nb = std::min<unsigned>(blocks_for(N, 256), 8192u);
+        const int ev = prof_start(g, SF_KERNEL_CFL);
         k_cfl<<<nb, 256, 0, g->stream>>>(g->G, g->buf[g->cur], cfl, &g->clock->dtmin, g->flags);
+        prof_stop(g, ev);
         SF_HIP(hipGetLastError());
     }
     g->dtmin_valid = 0;
@@ -288,6 +321,7 @@ int grid_free(sf_grid* g) {
     if (g->flags) hipFree(g->flags);
     if (g->dt_hist) hipFree(g->dt_hist);
     if (g->stage_io) hipFree(g->stage_io);
+    for (hipEvent_t e : g->ev_pool) hipEventDestroy(e);
     if (g->stream) hipStreamDestroy(g->stream);
     delete g;
     return 0;
@@ -625,6 +659,34 @@ int sf_error_flags(sf_grid* g, uint32_t* flags, int clear) {
         const unsigned zero = 0;
         SF_HIP(hipMemcpy(g->flags, &zero, sizeof zero, hipMemcpyHostToDevice));
     }
+    return 0;
+}
+
+int sf_profile_enable(sf_grid* g, int enable) {
+    if (!g) return fail(SF_E_ARG, "sf_profile_enable: null grid");
+    if (int rc = set_device(g)) return rc;
+    SF_HIP(hipStreamSynchronize(g->stream));
+    g->ev_used.clear();
+    g->profiling = enable ? 1 : 0;
+    return 0;
+}
+
+int sf_profile_read(sf_grid* g, int kernel, double* total_ms, long long* launches) {
+    if (!g || kernel < 0 || kernel >= SF_KERNEL_COUNT)
+        return fail(SF_E_ARG, "sf_profile_read: invalid argument");
+    if (int rc = set_device(g)) return rc;
+    SF_HIP(hipStreamSynchronize(g->stream));
+    double sum = 0.0;
+    long long n = 0;
+    for (const auto& u : g->ev_used) {
+        if (u.first != kernel) continue;
+        float ms = 0.0f;
+        SF_HIP(hipEventElapsedTime(&ms, g->ev_pool[u.second], g->ev_pool[u.second + 1]));
+        sum += ms;
+        ++n;
+    }
+    if (total_ms) *total_ms = sum;
+    if (launches) *launches = n;
     return 0;
 }
 

@@ -102,12 +102,19 @@ def tgv_3d(n: int = 512, seed: int = 2026, scheme="weno5", flux="hllc", time="ss
     h = 2.0 * math.pi / n
     ph = 2.0 * math.pi * uniform01(seed, 3)
     c = np.arange(n) * h
-    z, y, x = np.meshgrid(c, c, c, indexing="ij")
-    u = np.sin(x + ph[0]) * np.cos(y + ph[1]) * np.cos(z + ph[2])
-    v = -np.cos(x + ph[0]) * np.sin(y + ph[1]) * np.cos(z + ph[2])
-    rho = np.ones_like(x)
-    p = 100.0 + rho / 16.0 * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
-    q = conserved(rho, u, v, 0.0, p)
+    # separable factors broadcast over (z, y, x); the products are formed in
+    # the same order as the pointwise formula
+    x, y, z = c[None, None, :], c[None, :, None], c[:, None, None]
+    u = (np.sin(x + ph[0]) * np.cos(y + ph[1])) * np.cos(z + ph[2])
+    v = (-np.cos(x + ph[0]) * np.sin(y + ph[1])) * np.cos(z + ph[2])
+    p = 100.0 + 1.0 / 16.0 * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
+    q = np.empty((5, n, n, n))
+    q[0] = 1.0
+    q[1] = u
+    q[2] = v
+    q[3] = 0.0
+    q[4] = p / (GAMMA - 1.0) + 0.5 * 1.0 * (u * u + v * v + 0.0 * 0.0)
+    q = q.reshape(5, -1)
     desc = dict(n=(n, n, n), active=(1, 1, 1), bc=("periodic",) * 3, d=(h, h, h),
                 scheme=scheme, flux=flux, time=time)
     return desc, q
