@@ -2,13 +2,17 @@
 // gfx950: characteristic WENO5-JS / TENO5 reconstruction on an axis-aligned
 // face followed by a Rusanov or HLLC flux.
 //
-// The operation order follows the reference functions cited at each
+// The operations are those of the reference functions cited at each
 // definition (paths relative to the sonicSolver source tree).  On an
 // axis-aligned face the unit normal and the tangent basis (l, m) are
 // signed unit vectors, so every product with one of their zero components
 // is dropped and every product with +-1 becomes a sign; for finite inputs
-// this changes nothing but the sign of a zero.  hipcc may contract a*b+c
-// into FMA, so results differ from the reference in the last bits only.
+// this changes nothing but the sign of a zero.  FP64 division is a long
+// instruction sequence on CDNA, so where the reference divides several
+// times by one quantity the code takes one reciprocal, and the WENO
+// weights are normalised with a single division (each definition says
+// where).  With hipcc's FMA contraction, results differ from the
+// reference in the last few bits only.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -41,11 +45,18 @@ template <> struct Basis<2> {
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
 // WENO5::weno5_core, src/methods/numerics/convection/SF_WENO5.cpp:42-62.
+// Same weights and candidates, evaluated with one division: with
+// e_k = (eps + beta_k)^2 and alpha_k = d_k / e_k,
+//   sum_k alpha_k p_k / sum_k alpha_k
+//     = sum_k d_k (6 p_k) E_k / (6 sum_k d_k E_k),   E_k = prod_{j != k} e_j,
+// and the candidates are kept as 6 p_k.  e_k >= 1e-12, and for the
+// characteristic variables (O(rho)) the products stay far from overflow.
+// Rounding differs from the seven-division form by a few ulps.
 __device__ __forceinline__ double weno5(double v0, double v1, double v2,
                                         double v3, double v4) {
-    const double p0 = (2.0 * v0 - 7.0 * v1 + 11.0 * v2) / 6.0;
-    const double p1 = (-v1 + 5.0 * v2 + 2.0 * v3) / 6.0;
-    const double p2 = (2.0 * v2 + 5.0 * v3 - v4) / 6.0;
+    const double q0 = 2.0 * v0 - 7.0 * v1 + 11.0 * v2;
+    const double q1 = -v1 + 5.0 * v2 + 2.0 * v3;
+    const double q2 = 2.0 * v2 + 5.0 * v3 - v4;
     const double b0 = 13.0 / 12.0 * sq(v0 - 2.0 * v1 + v2)
                     + 0.25 * sq(v0 - 4.0 * v1 + 3.0 * v2);
     const double b1 = 13.0 / 12.0 * sq(v1 - 2.0 * v2 + v3)
@@ -53,21 +64,22 @@ __device__ __forceinline__ double weno5(double v0, double v1, double v2,
     const double b2 = 13.0 / 12.0 * sq(v2 - 2.0 * v3 + v4)
                     + 0.25 * sq(3.0 * v2 - 4.0 * v3 + v4);
     const double eps = 1e-6;
-    const double a0 = 0.1 / sq(eps + b0);
-    const double a1 = 0.6 / sq(eps + b1);
-    const double a2 = 0.3 / sq(eps + b2);
-    return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+    const double e0 = sq(eps + b0), e1 = sq(eps + b1), e2 = sq(eps + b2);
+    const double w0 = 0.1 * (e1 * e2), w1 = 0.6 * (e0 * e2), w2 = 0.3 * (e0 * e1);
+    return (w0 * q0 + w1 * q1 + w2 * q2) / (6.0 * (w0 + w1 + w2));
 }
 
 // TENO5::teno5_core, src/methods/numerics/convection/SF_TENO5.cpp:44-92
 // (C = 1, q = 6, CT = 1e-5).  gamma^6 is formed by three products instead
-// of std::pow; the two agree to a few ulps, which can only matter where
-// gamma_k / sum lands within that distance of CT.
+// of std::pow, and the cutoff test gamma_k / sum < CT is evaluated as
+// gamma_k < CT * sum; both agree with the reference to a few ulps, which
+// can only matter where gamma_k / sum lands within that distance of CT.
+// The 1/6 of the candidates is folded into the final division.
 __device__ __forceinline__ double teno5(double v0, double v1, double v2,
                                         double v3, double v4, unsigned& flags) {
-    const double p0 = (2.0 * v0 - 7.0 * v1 + 11.0 * v2) / 6.0;
-    const double p1 = (-v1 + 5.0 * v2 + 2.0 * v3) / 6.0;
-    const double p2 = (2.0 * v2 + 5.0 * v3 - v4) / 6.0;
+    const double q0 = 2.0 * v0 - 7.0 * v1 + 11.0 * v2;
+    const double q1 = -v1 + 5.0 * v2 + 2.0 * v3;
+    const double q2 = 2.0 * v2 + 5.0 * v3 - v4;
     const double b0 = 13.0 / 12.0 * sq(v0 - 2.0 * v1 + v2)
                     + 0.25 * sq(v0 - 4.0 * v1 + 3.0 * v2);
     const double b1 = 13.0 / 12.0 * sq(v1 - 2.0 * v2 + v3)
@@ -81,13 +93,13 @@ __device__ __forceinline__ double teno5(double v0, double v1, double v2,
     g0 = g0 * g0 * g0; g0 = g0 * g0;
     g1 = g1 * g1 * g1; g1 = g1 * g1;
     g2 = g2 * g2 * g2; g2 = g2 * g2;
-    const double gs = g0 + g1 + g2;
-    const double c0 = (g0 / gs < 1.0e-5) ? 0.0 : 1.0;
-    const double c1 = (g1 / gs < 1.0e-5) ? 0.0 : 1.0;
-    const double c2 = (g2 / gs < 1.0e-5) ? 0.0 : 1.0;
+    const double cut = 1.0e-5 * (g0 + g1 + g2);
+    const double c0 = (g0 < cut) ? 0.0 : 1.0;
+    const double c1 = (g1 < cut) ? 0.0 : 1.0;
+    const double c2 = (g2 < cut) ? 0.0 : 1.0;
     const double active = 0.1 * c0 + 0.6 * c1 + 0.3 * c2;
     if (!(active > 0.0)) flags |= kFlagTeno;
-    return (0.1 * c0 * p0 + 0.6 * c1 * p1 + 0.3 * c2 * p2) / active;
+    return (0.1 * c0 * q0 + 0.6 * c1 * q1 + 0.3 * c2 * q2) / (6.0 * active);
 }
 
 template <int SCHEME>
@@ -99,20 +111,22 @@ __device__ __forceinline__ double recon(double v0, double v1, double v2,
 
 // Closure of SingleFluidStateModel::close + PerfectGasEOS
 // (SF_singleFluidStateModel.cpp:49-71, SF_perfectGasEOS.h:24-32).
+// One reciprocal of the density replaces the five divisions by rho.
 struct Closed {
-    double u[3], p, c;
+    double u[3], p, c, inv_rho;
 };
 
 __device__ __forceinline__ Closed close_state(const double q[5], double gamma) {
     Closed s;
     const double rho = q[0];
-    s.u[0] = q[1] / rho;
-    s.u[1] = q[2] / rho;
-    s.u[2] = q[3] / rho;
+    s.inv_rho = 1.0 / rho;
+    s.u[0] = q[1] * s.inv_rho;
+    s.u[1] = q[2] * s.inv_rho;
+    s.u[2] = q[3] * s.inv_rho;
     const double kinetic = 0.5 * rho * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
-    const double e = (q[4] - kinetic) / rho;
+    const double e = (q[4] - kinetic) * s.inv_rho;
     s.p = (gamma - 1.0) * rho * e;
-    s.c = sqrt(gamma * s.p / rho);
+    s.c = sqrt(gamma * s.p * s.inv_rho);
     return s;
 }
 
@@ -155,7 +169,7 @@ __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
     double ut[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) ut[d] = (sL * l.u[d] + sR * r.u[d]) * inv;
-    const double HL = (qL[4] + l.p) / qL[0], HR = (qR[4] + r.p) / qR[0];
+    const double HL = (qL[4] + l.p) * l.inv_rho, HR = (qR[4] + r.p) * r.inv_rho;
     const double Ht = (sL * HL + sR * HR) * inv;
     const double ct = sqrt((gamma - 1.0) * (Ht - 0.5 * (ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2])));
     const double SL = fmin(unL - l.c, ut[ND] - ct);
@@ -186,7 +200,7 @@ __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
 #pragma unroll
     for (int d = 0; d < 3; ++d)
         us[1 + d] = scale * ((d == ND) ? s.u[d] + (Ss - un) : s.u[d]);
-    us[4] = scale * (q[4] / q[0] + (Ss - un) * (Ss + s.p / mK));
+    us[4] = scale * (q[4] * s.inv_rho + (Ss - un) * (Ss + s.p / mK));
 #pragma unroll
     for (int v = 0; v < 5; ++v) out[v] = f[v] + SK * (us[v] - q[v]);
 }
@@ -207,11 +221,10 @@ __device__ __forceinline__ void face_flux(const double st[6][5], double gamma,
     double qa[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) qa[v] = 0.5 * (st[2][v] + st[3][v]);
-    const double invRho0 = 1.0 / qa[0];
-    const double ke = 0.5 * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * invRho0;
+    const double invR = 1.0 / qa[0];
+    const double ke = 0.5 * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * invR;
     const double p = (gamma - 1.0) * (qa[4] - ke);
     if (!(qa[0] > 0.0) || !(p > 0.0)) flags |= kFlagChar;
-    const double invR = 1.0 / qa[0];
     double u[3];
     u[0] = qa[1] * invR;
     u[1] = qa[2] * invR;

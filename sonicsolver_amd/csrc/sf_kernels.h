@@ -196,7 +196,11 @@ __global__ __launch_bounds__(kXBlock) void k_sweep_x(GridDev G, const double* __
 
 // ---------------------------------------------------------------------------
 // y / z sweep: lane = x index, blockIdx.y = the remaining transverse index,
-// blockIdx.z = segment of SEG nodes along the sweep axis.
+// blockIdx.z = segment of SEG nodes along the sweep axis.  Each lane marches
+// along its column; the six-node stencil window lives in LDS (one private
+// slot set per lane, a ring indexed by `head`), so between faces only the
+// previous face flux stays in registers.  Lanes never read each other's
+// slots, so no barrier is needed.
 // ---------------------------------------------------------------------------
 constexpr int kMarchBlock = 64;
 
@@ -205,7 +209,9 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
                                                              const double* __restrict__ q,
                                                              double* __restrict__ S, StageArgs A,
                                                              int seg) {
-    const int i = blockIdx.x * kMarchBlock + threadIdx.x;
+    __shared__ double win[6][5][kMarchBlock];
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x * kMarchBlock + lane;
     const int o = blockIdx.y;
     const int a0 = blockIdx.z * seg;
     const int a1 = min(a0 + seg, G.n[AXIS]);
@@ -217,21 +223,29 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
         const long long step = G.stride[AXIS];
         // node (a0 - 3) along the axis
         long long id = node_index(G, i, j0, k0) - 3 * step;
+        for (int s = 0; s < 6; ++s)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) win[s][v][lane] = q[v * G.P + id + s * step];
         double st[6][5];
 #pragma unroll
         for (int s = 0; s < 6; ++s)
 #pragma unroll
-            for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + id + s * step];
+            for (int v = 0; v < 5; ++v) st[s][v] = win[s][v][lane];
         double fprev[5];
         face_flux<AXIS, SCHEME, FLUX>(st, G.gamma, G.area[AXIS], fprev, flags);
         id += 3 * step;  // node a0
+        int head = 0;    // slot holding stencil position 0
         for (int a = a0; a < a1; ++a, id += step) {
+            // drop the oldest node, append node a+3
 #pragma unroll
-            for (int s = 0; s < 5; ++s)
+            for (int v = 0; v < 5; ++v) win[head][v][lane] = q[v * G.P + id + 3 * step];
+            head = (head == 5) ? 0 : head + 1;
 #pragma unroll
-                for (int v = 0; v < 5; ++v) st[s][v] = st[s + 1][v];
+            for (int s = 0; s < 6; ++s) {
+                const int slot = (head + s >= 6) ? head + s - 6 : head + s;
 #pragma unroll
-            for (int v = 0; v < 5; ++v) st[5][v] = q[v * G.P + id + 3 * step];
+                for (int v = 0; v < 5; ++v) st[s][v] = win[slot][v][lane];
+            }
             double f[5];
             face_flux<AXIS, SCHEME, FLUX>(st, G.gamma, G.area[AXIS], f, flags);
             double s5[5];
