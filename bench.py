@@ -134,13 +134,13 @@ def main():
     axis, name = max(sweeps.items(), key=lambda kv: prof[kv[1]][0])
     total_ms, launches = prof[name]
     avg_ms = total_ms / max(launches, 1)
-    final = axis == 2
+    final = axis == 0  # the x sweep applies the stage update
     fl = flops.sweep_flops((n, n, n), axis, scheme, flux, final=final, dim=3, time=tscheme,
                            fold_dt=1.0 / 3.0 if final else 0.0)
     achieved_tf = fl / (avg_ms * 1e-3) / 1e12
-    kernel_names = {"sweep_x": "k_sweep_x (x faces, writes S)",
+    kernel_names = {"sweep_x": "k_sweep_x (x faces + stage update + CFL fold)",
                     "sweep_y": "k_sweep_march<1> (y faces, accumulates S)",
-                    "sweep_z": "k_sweep_march<2> (z faces + stage update + CFL fold)"}
+                    "sweep_z": "k_sweep_march<2> (z faces, writes S)"}
     traffic, traffic_src = traffic_measurement(name)
 
     step_flops = flops.flops_per_cell_step(3, scheme, flux, tscheme) * cells

@@ -153,6 +153,29 @@ def run(desc: Desc, q: np.ndarray, nsteps: int, cfl: float, max_dt: float = 1e30
         lib.sfo_grid_destroy(g)
 
 
+def run_driver(desc: Desc, q: np.ndarray, start: float, end: float, cfl: float,
+               max_dt: float = 1e300, end_step: int = 0, write_time: float = 1e300,
+               write_steps: int = 0, write_by_step: bool = False, cap: int = 100000):
+    """Time::Driver loop of the C restatement; returns (state, dt sequence)."""
+    lib = oracle()
+    lib.sfo_run_driver.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, _P, ctypes.c_int]
+    g = lib.sfo_grid_create(ctypes.byref(desc))
+    try:
+        a = _arr(q).reshape(5, -1).copy()
+        lib.sfo_set_state(g, _ptr(a))
+        dts = np.zeros(cap)
+        n = lib.sfo_run_driver(g, start, end, end_step, int(write_by_step), write_steps,
+                               write_time, cfl, max_dt, _ptr(dts), cap)
+        if n < 0:
+            raise OracleError(lib.sfo_error(g))
+        lib.sfo_get_state(g, _ptr(a))
+        return a, dts[:n]
+    finally:
+        lib.sfo_grid_destroy(g)
+
+
 def residual(desc: Desc, q: np.ndarray):
     lib = oracle()
     g = lib.sfo_grid_create(ctypes.byref(desc))

@@ -13,7 +13,8 @@
 // weno5_core / teno5_core / buildCharacteristicMatrix (SF_WENO5.cpp,
 // SF_TENO5.cpp, SF_riemann.cpp), RusanovEOS::faceFlux / deltaT
 // (SF_rusanovEOS.cpp), validateEquationState (SF_equationState.h) and
-// Time::Explicit::begin / executeStage (SF_explicit.cpp).
+// Time::Explicit::begin / executeStage (SF_explicit.cpp) and Time::Driver
+// (SF_time.cpp).
 //
 // What this file supplies itself:
 //  - the explicit tableaux, restated from SF_builtinProviders.cpp:85-127,
@@ -35,6 +36,7 @@
 #include "methods/numerics/structured/SF_residualAssembly.h"
 #include "models/physics/fluidStateModel/SF_factory.h"
 #include "solver/algorithm/time/SF_explicit.h"
+#include "solver/algorithm/time/SF_time.h"
 #include "solver/boundary/SF_applicator.h"
 
 #include "sfcpu.h"
@@ -324,6 +326,76 @@ int sfref_residual(const sfo_desc* d, const double* soa_in, double* soa_out) {
             }
   });
 }
+
+namespace {
+// One physical step on an assembled case: the body shared by sfref_run and
+// the time driver's advance callback.
+double step_case(Case& c, SF::State::StateBundle& state,
+                 const SF::System::CompiledTimeRecipe& recipe, double cfl, double max_dt,
+                 double limit) {
+    prepare_boundary(c);
+    double dt = SF::Numerics::RusanovEOS::deltaT(c.field, *c.eq, cfl);
+    dt = std::min({dt, max_dt, limit});
+    state.dt = dt;
+    SF::Time::Explicit::Workspace workspace;
+    SF::Time::Explicit::begin(workspace, state.patches, state, recipe, nullptr);
+    for (int st = 0; st < recipe.stageCount(); ++st) {
+        SF::Time::Explicit::executeStage(
+            workspace, st, state.patches, c.ws, state, nullptr,
+            [&](const std::vector<SF::Field*>&,
+                std::vector<SF::SolverAlgorithm::PatchWorkspace>&, double) { assemble(c); },
+            [&](const std::vector<SF::Field*>&) {},
+            [&](const std::vector<SF::Field*>& patches, const char* label) {
+                for (SF::Field* p : patches)
+                    SF::Numerics::validateEquationState(*p, *c.eq, label);
+            });
+    }
+    prepare_boundary(c);
+    state.time += dt;
+    ++state.step;
+    return dt;
+}
+}  // namespace
+
+extern "C" {
+
+// Time::Driver::run (SF_time.cpp:42-93) with the explicit step as its
+// advance callback; returns the number of steps.
+int sfref_run_driver(const sfo_desc* d, double* soa_io, double start, double end,
+                     int end_step, int write_by_step, int write_steps, double write_time,
+                     double cfl, double max_dt, double* dt_seq, int cap, int* nsteps) {
+  return guarded([&] {
+    auto c = make_case(*d, soa_io);
+    SF::State::StateBundle state;
+    state.patches = {&c->field};
+    state.stateModel = c->eq;
+    state.time = start;
+    const auto recipe = time_recipe(d->time);
+    SF::Time::RunControl rc;
+    rc.startTime = start;
+    rc.endTime = end;
+    rc.endStep = end_step;
+    rc.writeByStep = write_by_step != 0;
+    rc.writeIntervalSteps = write_steps;
+    rc.writeIntervalTime = write_time;
+    SF::Time::DriverCallbacks cb;
+    int n = 0;
+    cb.advance = [&](double limit, int) {
+        const double dt = step_case(*c, state, recipe, cfl, max_dt, limit);
+        if (dt_seq && n < cap) dt_seq[n] = dt;
+        ++n;
+        SF::Time::AdvanceResult r;
+        r.dt = dt;
+        r.time = state.time;
+        return r;
+    };
+    SF::Time::Driver(rc).run(cb);
+    *nsteps = n;
+    export_state(*c, soa_io);
+  });
+}
+
+}  // extern "C"
 
 // flow.step.prepare -> flow.dt.compute -> flow.step.begin ->
 // explicit.stage.execute x stages -> flow.step.commit, repeated nsteps

@@ -779,3 +779,44 @@ int sfo_run(sfo_grid *G, int nsteps, double cfl, double max_dt, double *dt_seq)
     }
     return 0;
 }
+
+/* Time::Driver::run (src/solver/algorithm/time/SF_time.cpp:42-93) over
+ * flow.step.prepare / flow.dt.compute (dt = min(CFL dt, maxDeltaT,
+ * limit), SF_flowOperations.cpp:487-497) / the step.  Steps until end_time
+ * (within 64 eps) or end_step; the limit is the time left to the end and,
+ * for time-based output, to the next output instant.  Returns the number
+ * of steps, or -1 on error. */
+int sfo_run_driver(sfo_grid *G, double start, double end, int end_step,
+                   int write_by_step, int write_steps, double write_time,
+                   double cfl, double max_dt, double *dt_seq, int cap)
+{
+    double time = start;
+    int step = 0;
+    double next_out = start + write_time;
+    int next_out_step = write_steps;
+    const double eps = DBL_EPSILON;
+    for (;;) {
+        const double remaining = end - time;
+        double scale = fabs(time) > fabs(end) ? fabs(time) : fabs(end);
+        if (scale < 1.0) scale = 1.0;
+        if (remaining <= 64.0 * eps * scale || (end_step > 0 && step >= end_step)) break;
+        ++step;
+        double limit = remaining;
+        if (!write_by_step && next_out - time < limit) limit = next_out - time;
+        if (!(limit > 0.0)) return -1;
+        double dt;
+        if (sfo_apply_bc(G) || sfo_compute_dt(G, cfl, &dt)) return -1;
+        if (max_dt < dt) dt = max_dt;
+        if (limit < dt) dt = limit;
+        if (dt_seq && step - 1 < cap) dt_seq[step - 1] = dt;
+        if (sfo_step(G, dt)) return -1;
+        const double new_time = time + dt;
+        double tscale = fabs(time) > fabs(new_time) ? fabs(time) : fabs(new_time);
+        if (tscale < 1.0) tscale = 1.0;
+        time = new_time;
+        if (write_by_step && step == next_out_step) next_out_step += write_steps;
+        else if (!write_by_step && time >= next_out - 1.0e-11 * tscale) next_out += write_time;
+    }
+    return step;
+}
+

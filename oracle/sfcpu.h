@@ -94,6 +94,10 @@ int sfo_step(sfo_grid *g, double dt);
 int sfo_run(sfo_grid *g, int nsteps, double cfl, double max_dt,
             double *dt_seq);
 unsigned sfo_error(const sfo_grid *g);
+/* Time::Driver::run semantics (end time, end step, output instants) */
+int sfo_run_driver(sfo_grid *g, double start, double end, int end_step,
+                   int write_by_step, int write_steps, double write_time,
+                   double cfl, double max_dt, double *dt_seq, int cap);
 
 #ifdef __cplusplus
 }

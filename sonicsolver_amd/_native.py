@@ -95,6 +95,7 @@ def load():
                 f"{LIB_PATH} is missing: build it with `python -m sonicsolver_amd.build` "
                 "(there is no CPU fallback)")
         _lib = _declare(ctypes.CDLL(str(LIB_PATH)))
+        _declare_host(_lib)
     return _lib
 
 
@@ -240,6 +241,131 @@ class Grid:
         b = ctypes.c_size_t(0)
         _check(load().sf_grid_device_bytes(self._h, ctypes.byref(b)))
         return b.value
+
+
+class RunControl(ctypes.Structure):
+    _fields_ = [("start_time", ctypes.c_double), ("end_time", ctypes.c_double),
+                ("end_step", ctypes.c_int), ("write_by_step", ctypes.c_int),
+                ("write_interval_steps", ctypes.c_int), ("write_interval_time", ctypes.c_double)]
+
+
+OUTPUT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                             ctypes.c_double)
+
+
+def _declare_host(lib):
+    sig = {
+        "sf_recipe_resolve": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+        "sf_recipe_from_pair": [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+        "sf_time_recipe_resolve": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)],
+        "sf_plan_describe": [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t],
+        "sf_solver_create": [ctypes.POINTER(GridDesc), ctypes.c_char_p, ctypes.c_char_p,
+                             ctypes.c_double, ctypes.c_double, ctypes.POINTER(_VP)],
+        "sf_solver_destroy": [_VP],
+        "sf_solver_grid": [_VP, ctypes.POINTER(_VP)],
+        "sf_solver_run": [_VP, ctypes.POINTER(RunControl), OUTPUT_FN, _VP,
+                          ctypes.POINTER(ctypes.c_int), _P],
+        "sf_solver_dt_history": [_VP, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+
+
+def resolve_recipe(name: str):
+    """(scheme, flux, halo width) of a convection term recipe name."""
+    s, f, h = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(load().sf_recipe_resolve(name.encode(), ctypes.byref(s), ctypes.byref(f),
+                                    ctypes.byref(h)))
+    return s.value, f.value, h.value
+
+
+def recipe_from_pair(scheme: str, flux: str):
+    s, f, h = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(load().sf_recipe_from_pair(scheme.encode(), flux.encode(), ctypes.byref(s),
+                                      ctypes.byref(f), ctypes.byref(h)))
+    return s.value, f.value, h.value
+
+
+def resolve_time_recipe(token: str):
+    t, n = ctypes.c_int(), ctypes.c_int()
+    _check(load().sf_time_recipe_resolve(token.encode(), ctypes.byref(t), ctypes.byref(n)))
+    return t.value, n.value
+
+
+def describe_plan(convection: str, time: str) -> str:
+    buf = ctypes.create_string_buffer(4096)
+    _check(load().sf_plan_describe(convection.encode(), time.encode(), buf, len(buf)))
+    return buf.value.decode()
+
+
+class Solver:
+    """Provider-level solver: recipe names, the compiled explicit plan and the
+    time-driver loop over one device grid."""
+
+    def __init__(self, geometry: GridDesc, convection: str, time: str, cfl: float,
+                 max_dt: float = 1.0e300):
+        lib = load()
+        h = _VP()
+        _check(lib.sf_solver_create(ctypes.byref(geometry), convection.encode(), time.encode(),
+                                    float(cfl), float(max_dt), ctypes.byref(h)))
+        self._h = h
+        g = _VP()
+        _check(lib.sf_solver_grid(h, ctypes.byref(g)))
+        self._grid = g
+        self.N = int(np.prod(list(geometry.n)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().sf_solver_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, soa: np.ndarray):
+        a = np.ascontiguousarray(soa, dtype=np.float64).reshape(5, self.N)
+        _check(load().sf_state_upload(self._grid, a.ctypes.data_as(_P)))
+
+    def download(self) -> np.ndarray:
+        out = np.empty((5, self.N), dtype=np.float64)
+        _check(load().sf_state_download(self._grid, out.ctypes.data_as(_P)))
+        return out
+
+    def run(self, end_time: float, start_time: float = 0.0, end_step: int = 0,
+            write_interval_time: float = 1e300, write_interval_steps: int = 0,
+            write_by_step: bool = False, on_output=None):
+        rc = RunControl(start_time, end_time, end_step, int(write_by_step),
+                        write_interval_steps, write_interval_time)
+        outputs = []
+
+        def cb(_user, step, t, dt):
+            outputs.append((step, t, dt))
+            if on_output is not None:
+                return 0 if on_output(step, t, dt) in (None, True) else 1
+            return 0
+
+        fn = OUTPUT_FN(cb)
+        steps = ctypes.c_int(0)
+        t = ctypes.c_double(0.0)
+        _check(load().sf_solver_run(self._h, ctypes.byref(rc), fn, None, ctypes.byref(steps),
+                                    ctypes.byref(t)))
+        return steps.value, t.value, outputs
+
+    def dt_history(self) -> np.ndarray:
+        n = ctypes.c_int(0)
+        _check(load().sf_solver_dt_history(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros(n.value)
+        _check(load().sf_solver_dt_history(self._h, out.ctypes.data_as(_P), n.value,
+                                           ctypes.byref(n)))
+        return out
 
 
 def describe_flags(flags: int) -> str:

@@ -34,6 +34,17 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr int kMarchSeg = 32;
+
+}  // namespace
+
+namespace sfamd {
+// Used by the provider layer (host/sf_provider.cpp) to report through
+// sf_last_error.
+void set_last_error(const std::string& message) { g_last_error = message; }
+}  // namespace sfamd
+
+namespace {
+
 constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;
 
 }  // namespace
@@ -167,7 +178,11 @@ void launch_axis(sf_grid* g, int axis, const double* q, const StageArgs& A) {
     const GridDev& G = g->G;
     const int ev = prof_start(g, SF_KERNEL_SWEEP_X + axis);
     if (axis == 0) {
-        dim3 grid(blocks_for(G.n[0], kXBlock - 1), G.n[1], G.n[2]);
+        // a few thousand blocks in total, each looping over x lines
+        const unsigned bx = blocks_for(G.n[0], kXBlock - 1);
+        const unsigned lines = (unsigned)G.n[1] * (unsigned)G.n[2];
+        const unsigned by = std::max(1u, std::min(lines, 16384u / bx));
+        dim3 grid(bx, by, 1);
         k_sweep_x<SCHEME, FLUX, MODE><<<grid, kXBlock, 0, g->stream>>>(G, q, g->S, A);
         prof_stop(g, ev);
         return;
@@ -183,10 +198,15 @@ void launch_axis(sf_grid* g, int axis, const double* q, const StageArgs& A) {
     prof_stop(g, ev);
 }
 
+// Sweeps run z, y, then x: the x sweep (one face per lane) applies the
+// stage update, because its register footprint is the smallest and the
+// update adds to it.  The flux differences are therefore summed as
+// (dz + dy) + dx rather than the reference's (dx + dy) + dz, which changes
+// the last bit at most.
 template <int SCHEME, int FLUX>
 void launch_sweeps(sf_grid* g, const double* q, const StageArgs& A) {
     int axes[3], na = 0;
-    for (int a = 0; a < 3; ++a)
+    for (int a = 2; a >= 0; --a)
         if (g->G.active[a]) axes[na++] = a;
     for (int t = 0; t < na; ++t) {
         if (t == na - 1) launch_axis<SCHEME, FLUX, kFinal>(g, axes[t], q, A);

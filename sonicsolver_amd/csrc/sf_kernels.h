@@ -13,12 +13,12 @@
 //                                               in registers, so each node is
 //                                               loaded once and each face
 //                                               evaluated once
-//   the last active sweep also applies the stage update, validates the new
+//   the last sweep (x) also applies the stage update, validates the new
 //   state and (on the last stage) folds the CFL scan of the next step into
 //   a wave-level minimum plus one 64-bit atomic per wave.
 //
-// Sweeps accumulate S = sum_d (F_d(i) - F_d(i-1)) in the order x, y, z and
-// the update multiplies by Jac once, as Math::localSpatialResidual does
+// Sweeps accumulate S = sum_d (F_d(i) - F_d(i-1)) and the update multiplies
+// by Jac once, as Math::localSpatialResidual does
 // (SF_residualAssembly.h:12-32).
 #pragma once
 
@@ -144,54 +144,82 @@ __device__ __forceinline__ void stage_update(const GridDev& G, const StageArgs& 
 
 // ---------------------------------------------------------------------------
 // x sweep: lane t of a 128-lane block evaluates face (i0-1+t)+1/2; lanes
-// 1..127 then form the difference for node i0-1+t.  Blocks cover 127 nodes.
+// 1..127 then form the difference for node i0-1+t.  Blocks cover 127 nodes
+// of an x line and loop over lines (blockIdx.y, stride gridDim.y), so the
+// CFL fold costs one atomic per block rather than per line.
 // ---------------------------------------------------------------------------
 constexpr int kXBlock = 128;
+
+// Block-wide minimum of a positive double into *target (one atomic).
+__device__ __forceinline__ void block_min_atomic(double v, unsigned long long* target,
+                                                 unsigned long long* lds) {
+    unsigned long long b = (v > 0.0 && isfinite(v)) ? (unsigned long long)__double_as_longlong(v)
+                                                    : 0x7FF0000000000000ull;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = o < b ? o : b;
+    }
+    const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[wave] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nwave; ++w) b = lds[w] < b ? lds[w] : b;
+        if (b != 0x7FF0000000000000ull) atomicMin(target, b);
+    }
+}
 
 template <int SCHEME, int FLUX, int MODE>
 __global__ __launch_bounds__(kXBlock) void k_sweep_x(GridDev G, const double* __restrict__ q,
                                                      double* __restrict__ S, StageArgs A) {
     __shared__ double sh[5][kXBlock];
+    __shared__ unsigned long long red[kXBlock / 64];
     const int t = threadIdx.x;
     const int i0 = blockIdx.x * (kXBlock - 1);
-    const int j = blockIdx.y, k = blockIdx.z;
     const int a = i0 - 1 + t;  // lower node of this lane's face
+    const int lines = G.n[1] * G.n[2];
     unsigned flags = 0;
-    if (a <= G.n[0] - 1) {
-        const long long base = node_index(G, a, j, k);
-        double st[6][5];
-#pragma unroll
-        for (int s = 0; s < 6; ++s)
-#pragma unroll
-            for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + base + (s - 2)];
-        double f[5];
-        face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f, flags);
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            if (!isfinite(f[v])) flags |= kFlagFlux;
-            sh[v][t] = f[v];
-        }
-    }
-    __syncthreads();
     double dt_local = INFINITY;
-    if (t >= 1 && a <= G.n[0] - 1) {
-        const long long id = node_index(G, a, j, k);
-        double s5[5];
+    for (int line = blockIdx.y; line < lines; line += gridDim.y) {
+        const int j = line % G.n[1], k = line / G.n[1];
+        if (a <= G.n[0] - 1) {
+            const long long base = node_index(G, a, j, k);
+            double st[6][5];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            const double d = sh[v][t] - sh[v][t - 1];
-            s5[v] = (MODE == kCopyS) ? d : S[v * G.P + id] + d;
-        }
-        if (MODE == kFinal) {
-            if (solved_node(G, a, j, k)) stage_update(G, A, id, q, s5, flags, dt_local);
-            else carry_node(G, A, id, q);
-        } else {
+            for (int s = 0; s < 6; ++s)
 #pragma unroll
-            for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
+                for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + base + (s - 2)];
+            double f[5];
+            face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f, flags);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                if (!isfinite(f[v])) flags |= kFlagFlux;
+                sh[v][t] = f[v];
+            }
         }
+        __syncthreads();
+        if (t >= 1 && a <= G.n[0] - 1) {
+            const long long id = node_index(G, a, j, k);
+            double s5[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                const double d = sh[v][t] - sh[v][t - 1];
+                s5[v] = (MODE == kCopyS) ? d : S[v * G.P + id] + d;
+            }
+            if (MODE == kFinal) {
+                double dl = INFINITY;
+                if (solved_node(G, a, j, k)) stage_update(G, A, id, q, s5, flags, dl);
+                else carry_node(G, A, id, q);
+                dt_local = fmin(dt_local, dl);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
+            }
+        }
+        __syncthreads();  // sh is rewritten by the next line
     }
     raise_flags(flags, A.flags);
-    if (MODE == kFinal && A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);
+    if (MODE == kFinal && A.reduce_dt) block_min_atomic(dt_local, A.dtmin, red);
 }
 
 // ---------------------------------------------------------------------------

@@ -116,3 +116,28 @@ def test_residual_bit_exact():
         q = wl.smooth_field(n, seed=5)
         desc = ob.make_desc(n, active, bc, (0.1, 0.12, 0.09), scheme="teno5", flux="hllc")
         assert np.array_equal(ob.residual(desc, q), ob.ref_residual(desc, q))
+
+
+@pytest.mark.parametrize("end,end_step,by_step,wsteps,wtime", [
+    (0.05, 0, 0, 0, 0.02), (0.05, 0, 1, 7, 1e300), (1.0, 13, 0, 0, 0.3)])
+def test_time_driver_bit_exact(end, end_step, by_step, wsteps, wtime):
+    """The oracle's Time::Driver loop against the reference's own
+    (SF_time.cpp:42-93): end time, output instants, step outputs, end step."""
+    import ctypes
+    ref = ob.ref()
+    P = ctypes.POINTER(ctypes.c_double)
+    ref.sfref_run_driver.argtypes = [
+        ctypes.POINTER(ob.Desc), P, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, P, ctypes.c_int,
+        ctypes.POINTER(ctypes.c_int)]
+    kw, q = wl.sod_1d(200)
+    a = q.copy()
+    d = np.zeros(2000)
+    n = ctypes.c_int()
+    assert ref.sfref_run_driver(ctypes.byref(ob.make_desc(**kw)), a.ctypes.data_as(P), 0.0, end,
+                                end_step, by_step, wsteps, wtime, 0.5, 1e300,
+                                d.ctypes.data_as(P), 2000, ctypes.byref(n)) == 0
+    b, bd = ob.run_driver(ob.make_desc(**kw), q, 0.0, end, 0.5, end_step=end_step,
+                          write_by_step=bool(by_step), write_steps=wsteps, write_time=wtime)
+    assert n.value == len(bd)
+    assert np.array_equal(d[:n.value], bd) and np.array_equal(a, b)
