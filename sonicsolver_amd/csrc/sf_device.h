@@ -1,0 +1,180 @@
+// sf_device.h -- device-side pieces shared by the grid kernels
+// (sf_kernels.h) and the batched kernels (sf_batch.hip): grid descriptor,
+// stage-update arithmetic, validation flags, CFL folding and the pressure /
+// energy conversions of the boundary closure.
+#pragma once
+
+#include "sf_face.h"
+
+namespace sfamd {
+
+struct GridDev {
+    int n[3];
+    int g[3];
+    int m[3];
+    long long P;        // padded nodes per variable
+    long long stride[3];
+    int active[3];
+    int bc[3];          // 0 transmissive, 1 periodic
+    double jac, area[3], metric[3], gamma;
+};
+
+enum { kCopyS = 0, kAccumS = 1, kFinal = 2 };
+
+// Stage update parameters (Time::Explicit, SF_explicit.cpp:193-238).
+struct StageArgs {
+    const double* q0;        // step-start state (SSPRK3, RK4)
+    double* qout;            // new state
+    double* k[3];            // RK4 stage residuals k1..k3
+    const double* dt;        // device scalar: dt of this step
+    unsigned long long* dtmin;  // CFL accumulator (bits of a positive double)
+    unsigned* flags;
+    double base, incr;       // SSPRK3 base / increment weights
+    double cfl;
+    int time;                // 0 Euler, 1 SSPRK3, 2 RK4
+    int stage;
+    int reduce_dt;           // fold the next step's CFL scan into this update
+};
+
+__device__ __forceinline__ long long node_index(const GridDev& G, int i, int j, int k) {
+    return (long long)(k + G.g[2]) * G.stride[2] + (long long)(j + G.g[1]) * G.stride[1]
+         + (i + G.g[0]);
+}
+
+__device__ __forceinline__ bool solved_node(const GridDev& G, int i, int j, int k) {
+    const int p[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (G.active[d] && G.bc[d] == 0 && (p[d] == 0 || p[d] == G.n[d] - 1)) return false;
+    return true;
+}
+
+// Local CFL step of RusanovEOS::deltaT (SF_rusanovEOS.cpp:110-147).  GT is
+// any descriptor with gamma, active[3] and metric[3].
+template <class GT>
+__device__ __forceinline__ double local_dt(const GT& G, const double q[5], double cfl,
+                                           unsigned& flags) {
+    const Closed s = close_state(q, G.gamma);
+    double spectral = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (G.active[d]) spectral += fabs(s.u[d] * G.metric[d]) + s.c * G.metric[d];
+    if (!(spectral > 0.0) || !isfinite(spectral)) flags |= kFlagDt;
+    return cfl / spectral;
+}
+
+__device__ __forceinline__ void wave_min_atomic(double v, unsigned long long* target) {
+    // positive doubles order like their bit patterns
+    unsigned long long b = (v > 0.0 && isfinite(v)) ? (unsigned long long)__double_as_longlong(v)
+                                                    : 0x7FF0000000000000ull;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = o < b ? o : b;
+    }
+    if ((threadIdx.x & 63) == 0 && b != 0x7FF0000000000000ull) atomicMin(target, b);
+}
+
+__device__ __forceinline__ void raise_flags(unsigned f, unsigned* flags) {
+    if (f) atomicOr(flags, f);
+}
+
+// The reference updates the state in place, so a node that the stage does
+// not advance (a solver-boundary node) keeps the value of the last ghost
+// fill; with separate input and output buffers it is copied across.
+template <class GT>
+__device__ __forceinline__ void carry_node(const GT& G, const StageArgs& A, long long id,
+                                           const double* __restrict__ q) {
+    if (A.time == 3) return;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) A.qout[v * G.P + id] = q[v * G.P + id];
+}
+
+// Stage update of one node; S is the accumulated flux difference.  GT is
+// any descriptor with P, jac, gamma, active[3] and metric[3]; the step's dt
+// is read from *A.dt.
+template <class GT>
+__device__ __forceinline__ void stage_update(const GT& G, const StageArgs& A, long long id,
+                                             const double* __restrict__ q, const double S[5],
+                                             unsigned& flags, double& dt_local) {
+    if (A.time == 3) {  // residual only (diagnostics)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) A.qout[v * G.P + id] = G.jac * S[v];
+        return;
+    }
+    const double dt = *A.dt;
+    double qn[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const double R = G.jac * S[v];
+        const double qc = q[v * G.P + id];
+        if (A.time == 1) {
+            const double euler = qc - dt * R;
+            qn[v] = (A.stage == 0 ? 0.0 : A.base * A.q0[v * G.P + id]) + A.incr * euler;
+        } else if (A.time == 0) {
+            qn[v] = qc - dt * R;
+        } else if (A.stage < 3) {
+            A.k[A.stage][v * G.P + id] = R;
+            qn[v] = A.q0[v * G.P + id] - A.incr * dt * R;
+        } else {
+            qn[v] = A.q0[v * G.P + id]
+                  - dt / 6.0 * (A.k[0][v * G.P + id] + 2.0 * A.k[1][v * G.P + id]
+                                + 2.0 * A.k[2][v * G.P + id] + R);
+        }
+        A.qout[v * G.P + id] = qn[v];
+    }
+    // validateEquationState (SF_equationState.h:25-57): rho, p > 0, finite
+    const double ke = 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) / qn[0];
+    const double eint = qn[4] - ke;
+    if (!(qn[0] > 0.0) || !(eint > 0.0) || !isfinite(eint)) flags |= kFlagState;
+    if (A.reduce_dt) dt_local = local_dt(G, qn, A.cfl, flags);
+}
+
+// Block-wide minimum of a positive double into *target (one atomic).
+__device__ __forceinline__ void block_min_atomic(double v, unsigned long long* target,
+                                                 unsigned long long* lds) {
+    unsigned long long b = (v > 0.0 && isfinite(v)) ? (unsigned long long)__double_as_longlong(v)
+                                                    : 0x7FF0000000000000ull;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = o < b ? o : b;
+    }
+    const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[wave] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nwave; ++w) b = lds[w] < b ? lds[w] : b;
+        if (b != 0x7FF0000000000000ull) atomicMin(target, b);
+    }
+}
+
+// Device clock: dt of the step, physical time, step count, CFL accumulator
+// (bits of a positive double) and the dt limits.
+struct Clock {
+    double dt;
+    double time;
+    long long steps;
+    unsigned long long dtmin;
+    double max_dt;
+    double limit;
+};
+
+__device__ __forceinline__ double pressure_of(const double* q, long long P, long long id,
+                                              double gamma) {
+    double qq[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qq[v] = q[v * P + id];
+    return close_state(qq, gamma).p;
+}
+
+// totalEnergyFromPressure (SF_singleFluidStateModel.cpp:73-109).
+__device__ __forceinline__ double energy_from_pressure(const double* q, long long P, long long id,
+                                                       double gamma, double p) {
+    const double r = q[id], m1 = q[P + id], m2 = q[2 * P + id], m3 = q[3 * P + id];
+    const double e = 0.0 + p / ((gamma - 1.0) * r);
+    const double kinetic = 0.5 * (1.0 / r) * (m1 * m1 + m2 * m2 + m3 * m3);
+    return r * e + kinetic;
+}
+
+}  // namespace sfamd
