@@ -172,6 +172,68 @@ enum {
 int sf_profile_enable(sf_grid *grid, int enable);
 int sf_profile_read(sf_grid *grid, int kernel, double *total_ms, long long *launches);
 
+/* ---- batched solves ----------------------------------------------------------
+ * No reference counterpart: the reference advances one case per process
+ * (app/application/SF_application.cpp:111-118 runs one case).  A batch
+ * holds many independent 1D or 2D problems of uneven size on one device:
+ * each has its own scheme, flux, CFL number, dt and clock, transmissive
+ * boundaries on every side (the reference's zeroGradient sets,
+ * SF_zeroGradient.cpp:88-130) and SSPRK3 time stepping
+ * (SF_builtinProviders.cpp:98-121).  One step of the batch advances every
+ * live problem by one SSPRK3 step with its own CFL dt; problems never
+ * interact, so each result is the same whichever problems share the batch.
+ * State crosses the boundary per problem as soa[v*N + j*nx + i],
+ * N = nx*ny. */
+typedef struct {
+    int n[2];        /* unique nodes along x and y; n[1] = 1 for a 1D problem */
+    double d[2];     /* node spacing (d[1] ignored for a 1D problem)           */
+    int scheme;      /* SF_SCHEME_*                                            */
+    int flux;        /* SF_FLUX_*                                              */
+    double cfl;      /* CFL number of this problem's dt                        */
+} sf_batch_item;
+
+typedef struct sf_batch sf_batch;
+
+/* Padded device nodes a problem occupies in the arena (per variable); the
+ * arena holds 4 values of 5 variables per node (three state buffers and the
+ * flux accumulator), 160 bytes. */
+int sf_batch_item_nodes(const sf_batch_item *item, size_t *nodes);
+/* A batch with an arena of node_capacity padded nodes and up to
+ * item_capacity problems. */
+int sf_batch_create(int device, double gamma, size_t node_capacity, int item_capacity,
+                    sf_batch **out);
+int sf_batch_destroy(sf_batch *batch);
+/* Admit a problem: first-fit placement of its footprint in the arena
+ * (SF_E_MEMORY when it does not fit).  max_steps > 0 freezes the problem
+ * after that many steps; 0 leaves it running until removed.  The slot
+ * numbers the problem until sf_batch_remove. */
+int sf_batch_add(sf_batch *batch, const sf_batch_item *item, const double *soa,
+                 long long max_steps, int *slot);
+int sf_batch_remove(sf_batch *batch, int slot);
+int sf_batch_download(sf_batch *batch, int slot, double *soa);
+/* Advance every live problem by nsteps steps; use_graph replays one
+ * captured hipGraph per step (captured again after admissions/removals). */
+int sf_batch_step(sf_batch *batch, int nsteps, int use_graph);
+/* Physical time and step count of one problem. */
+int sf_batch_clock(sf_batch *batch, int slot, double *time, long long *steps);
+/* Step counts of all slots (item_capacity entries; -1 for a free slot). */
+int sf_batch_steps(sf_batch *batch, long long *steps);
+/* Arena occupancy in padded nodes, and the number of admitted problems. */
+int sf_batch_memory(const sf_batch *batch, size_t *used_nodes, size_t *free_nodes,
+                    int *items);
+/* Tile shape of the persistent sweeps: x tiles of xlen nodes of one line
+ * (63, 127 or 255), y tiles of 64 columns by ylen nodes (4..4096).  Takes
+ * effect at the next step; results do not depend on it. */
+int sf_batch_set_tiles(sf_batch *batch, int xlen, int ylen);
+/* Persistent grid size of the sweeps in workgroups per compute unit
+ * (1..64).  Results do not depend on it. */
+int sf_batch_set_occupancy(sf_batch *batch, int blocks_per_cu);
+/* Sweep schedule: fused != 0 runs one launch per sweep for every
+ * (scheme, flux) variant (workgroup ranges assigned per variant); 0 runs one
+ * launch per variant present (fewer registers per kernel).  Results do not
+ * depend on it. */
+int sf_batch_set_schedule(sf_batch *batch, int fused);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,7 +18,8 @@ ROOT = PKG.parent
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SONIC_AMD_ARCH", "gfx950")
 
-SOURCES = [PKG / "csrc" / "sf_capi.hip", PKG / "host" / "sf_provider.cpp"]
+SOURCES = [PKG / "csrc" / "sf_capi.hip", PKG / "csrc" / "sf_batch.hip",
+           PKG / "host" / "sf_provider.cpp", PKG / "host" / "sf_server.cpp"]
 HEADERS = list((PKG / "csrc").glob("*.h")) + list((PKG / "host").glob("*.h")) + \
     list((ROOT / "include").glob("*.h"))
 OUTPUT = PKG / "libsonic_amd.so"
@@ -31,14 +32,34 @@ def hip_flags():
 
 
 def build(verbose: bool = False) -> Path:
+    """Compile each source to an object in _build/ (in parallel), then link."""
+    objdir = ROOT / "_build" / "sonic_amd"
+    objdir.mkdir(parents=True, exist_ok=True)
     srcs = [s for s in SOURCES if s.exists()]
-    cmd = [HIPCC, *hip_flags(), "-shared", "-o", str(OUTPUT) + ".tmp", *map(str, srcs)]
+    procs = []
+    for src in srcs:
+        obj = objdir / (src.name + ".o")
+        cmd = [HIPCC, *hip_flags(), "-c", "-o", str(obj), str(src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((obj, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                            stderr=subprocess.STDOUT, text=True)))
+    failed = False
+    for obj, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(OUTPUT) + ".tmp",
+           *(str(o) for o, _ in procs)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with status {res.returncode}")
+        raise RuntimeError(f"hipcc link failed with status {res.returncode}")
     os.replace(str(OUTPUT) + ".tmp", OUTPUT)
     return OUTPUT
 

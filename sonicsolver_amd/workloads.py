@@ -132,3 +132,78 @@ def smooth_field(n, seed: int = 7, amp: float = 0.2):
     w = 0.2 * np.cos(tz + 6 * r[5])
     p = 1.0 + amp * np.cos(tx + ty)
     return conserved(rho, u, v, w, p)
+
+
+def _loguniform(u, lo, hi):
+    return np.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))
+
+
+def riemann_1d(cells: int, left, right, interface: float):
+    """1D Riemann problem on x in [0, 1] with cells+1 nodes; node x_i = i/cells
+    takes the left (rho, u, p) where x_i < interface.  Returns (item kwargs
+    without scheme/flux, q)."""
+    nx = cells + 1
+    x = np.arange(nx) / cells
+    lft = x < interface
+    rho = np.where(lft, left[0], right[0])
+    u = np.where(lft, left[1], right[1])
+    p = np.where(lft, left[2], right[2])
+    return dict(n=(nx, 1), d=(1.0 / cells, 1.0)), conserved(rho, u, 0.0, 0.0, p)
+
+
+def batch_riemann_1d(count: int = 4096, seed: int = 4096, lo: int = 256, hi: int = 8192,
+                     cfl: float = 0.5):
+    """Independent 1D Riemann problems of uneven length: L = round(exp(U[ln lo,
+    ln hi])) cells; rho log-uniform [0.1, 2], u uniform [-1, 1], p log-uniform
+    [0.1, 10] on each side; interface uniform in [0.3, 0.7]; problem i uses
+    WENO5 for even i, TENO5 for odd i, and Rusanov / HLLC alternating with
+    (i >> 1) & 1.  Yields (index, item kwargs, q)."""
+    r = uniform01(seed, 8 * count).reshape(count, 8)
+    for i in range(count):
+        cells = int(round(float(_loguniform(r[i, 0], lo, hi))))
+        left = (float(_loguniform(r[i, 1], 0.1, 2.0)), -1.0 + 2.0 * r[i, 2],
+                float(_loguniform(r[i, 3], 0.1, 10.0)))
+        right = (float(_loguniform(r[i, 4], 0.1, 2.0)), -1.0 + 2.0 * r[i, 5],
+                 float(_loguniform(r[i, 6], 0.1, 10.0)))
+        x0 = 0.3 + 0.4 * r[i, 7]
+        kw, q = riemann_1d(cells, left, right, x0)
+        kw.update(scheme="teno5" if i & 1 else "weno5",
+                  flux="hllc" if (i >> 1) & 1 else "rusanov", cfl=cfl)
+        yield i, kw, q
+
+
+def quadrant_2d(n: int, states):
+    """Four-quadrant 2D Riemann problem on [0, 1]^2 with n x n nodes; states
+    are (rho, u, v, p) of the quadrants (x>=1/2, y>=1/2), (x<1/2, y>=1/2),
+    (x<1/2, y<1/2), (x>=1/2, y<1/2), counter-clockwise from the upper right.
+    Returns (item kwargs without scheme/flux, q)."""
+    h = 1.0 / (n - 1)
+    x = np.arange(n) * h
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    right, top = X >= 0.5, Y >= 0.5
+    quad = np.where(top, np.where(right, 0, 1), np.where(right, 3, 2)).ravel()
+    st = np.asarray(states, dtype=np.float64)
+    rho, u, v, p = (st[quad, k] for k in range(4))
+    return dict(n=(n, n), d=(h, h)), conserved(rho, u, v, 0.0, p)
+
+
+def quadrant_requests(count: int, seed: int = 5005, lo: int = 64, hi: int = 1024,
+                      steps=(50, 500), cfl: float = 0.5):
+    """Served 2D request stream: N = round(exp(U[ln lo, ln hi])) nodes per
+    side, quadrant (rho, u, v, p) with rho in [0.1, 2], u, v in [-1, 1],
+    p in [0.1, 2] (uniform), steps uniform in the closed range `steps`;
+    WENO5 / HLLC.  Yields (index, item kwargs, steps, q-factory) -- the
+    state is built on demand, the largest is 1024^2 x 5 doubles."""
+    r = uniform01(seed, 18 * count).reshape(count, 18)
+    for i in range(count):
+        n = int(round(float(_loguniform(r[i, 0], lo, hi))))
+        states = []
+        for k in range(4):
+            a = r[i, 1 + 4 * k: 5 + 4 * k]
+            states.append((0.1 + 1.9 * a[0], -1.0 + 2.0 * a[1], -1.0 + 2.0 * a[2],
+                           0.1 + 1.9 * a[3]))
+        nsteps = int(steps[0] + min(int(r[i, 17] * (steps[1] - steps[0] + 1)),
+                                    steps[1] - steps[0]))
+        kw = dict(n=(n, n), d=(1.0 / (n - 1), 1.0 / (n - 1)), scheme="weno5", flux="hllc",
+                  cfl=cfl)
+        yield i, kw, nsteps, (lambda n=n, states=states: quadrant_2d(n, states)[1])

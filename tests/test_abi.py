@@ -12,7 +12,7 @@ HEADER = ROOT / "include" / "sonic_amd.h"
 
 
 def declared():
-    text = HEADER.read_text()
+    text = "".join(h.read_text() for h in sorted((ROOT / "include").glob("*.h")))
     return sorted(set(re.findall(r"^int\s+(sf_\w+)\s*\(", text, flags=re.M)))
 
 
@@ -82,3 +82,26 @@ def test_oracle_exports_every_declared_symbol():
     lib = ob.oracle()
     for n in names:
         assert hasattr(lib, n), n
+
+
+def test_batch_items_checked_on_host():
+    from sonicsolver_amd.batch import make_item, item_nodes
+    from sonicsolver_amd import SonicError
+    assert item_nodes(make_item((1001,), (1e-3,))) == 1007
+    assert item_nodes(make_item((64, 64), (0.1, 0.1))) == 70 * 70
+    for args in [dict(n=(3,), d=(0.1,)), dict(n=(16, 2), d=(0.1, 0.1)),
+                 dict(n=(16,), d=(0.0,)), dict(n=(16,), d=(0.1,), scheme=3),
+                 dict(n=(16,), d=(0.1,), flux=9), dict(n=(16,), d=(0.1,), cfl=0.0)]:
+        with pytest.raises(SonicError) as e:
+            item_nodes(make_item(**args))
+        assert e.value.code == -1
+
+
+def test_batch_null_handles_rejected():
+    from sonicsolver_amd import batch
+    lib = batch._lib()
+    assert lib.sf_batch_create(0, 1.4, 1000, 4, None) == -1
+    assert lib.sf_batch_step(None, 1, 1) == -1
+    assert lib.sf_batch_remove(None, 0) == -1
+    assert lib.sf_batch_set_tiles(None, 127, 32) == -1
+    assert lib.sf_batch_destroy(None) == 0

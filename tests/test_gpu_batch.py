@@ -1,0 +1,153 @@
+"""Batched solves on the device against the oracle, and the properties the
+batch promises: each problem's result does not depend on which problems
+share the batch (bit for bit), nor on the launch schedule, tile shape,
+persistent grid size or graph replay; admission by arena footprint; step
+limits.  Tolerance against the oracle: relative L1 and Linf <= 1e-11."""
+import numpy as np
+import pytest
+
+from oracle import bind as ob
+from sonicsolver_amd import Grid, make_desc, workloads as wl
+from sonicsolver_amd.batch import Batch, make_item, item_nodes
+from tests.parity import assert_close
+
+pytestmark = pytest.mark.gpu
+
+QUADS = [(0.5, 0.1, -0.2, 1.0), (1.0, -0.3, 0.2, 0.4), (0.2, 0.5, 0.5, 0.3), (1.5, 0.0, 0.1, 1.7)]
+
+
+def _oracle(kw, q, steps):
+    n = kw["n"]
+    two_d = len(n) > 1 and n[1] > 1
+    desc = ob.make_desc(n=(n[0], n[1] if two_d else 1, 1), active=(1, 1 if two_d else 0, 0),
+                        d=(kw["d"][0], kw["d"][1] if two_d else 1.0, 1.0),
+                        scheme=kw["scheme"], flux=kw["flux"], time="ssprk3")
+    return ob.run(desc, q, steps, kw["cfl"])
+
+
+def _mixed_problems(count=12, seed=7, lo=40, hi=700):
+    probs = [(kw, q) for _, kw, q in wl.batch_riemann_1d(count, seed=seed, lo=lo, hi=hi)]
+    for n, scheme, flux in [(9, "teno5", "hllc"), (33, "weno5", "hllc"), (70, "weno5", "rusanov"),
+                            (129, "teno5", "rusanov")]:
+        kw, q = wl.quadrant_2d(n, QUADS)
+        kw.update(scheme=scheme, flux=flux, cfl=0.5)
+        probs.append((kw, q))
+    return probs
+
+
+def _run(probs, steps, graph=True, fused=True, tiles=(127, 32), occupancy=8, capacity=None):
+    cap = capacity or sum(item_nodes(make_item(**kw)) for kw, _ in probs) + 64
+    with Batch(cap, len(probs) + 1) as b:
+        b.set_schedule(fused)
+        b.set_tiles(*tiles)
+        b.set_occupancy(occupancy)
+        slots = [b.add(make_item(**kw), q) for kw, q in probs]
+        b.step(steps, graph=graph)
+        outs = [b.download(s) for s in slots]
+        clocks = [b.clock(s) for s in slots]
+    return outs, clocks
+
+
+def test_batch_against_oracle():
+    probs = _mixed_problems()
+    steps = 25
+    print(f"batch of {len(probs)} problems, {steps} steps", flush=True)
+    outs, clocks = _run(probs, steps)
+    for (kw, q), out, (t, k) in zip(probs, outs, clocks):
+        ref, dts = _oracle(kw, q, steps)
+        assert k == steps
+        assert abs(t - dts.sum()) <= 1e-12 * dts.sum()
+        assert_close(out, ref, what=f"batch problem n={kw['n']} {kw['scheme']}/{kw['flux']}")
+
+
+def test_batch_riemann_full_length_range():
+    """Lengths 256..8192 cells as in the batched configuration, 60 steps."""
+    probs = [(kw, q) for _, kw, q in wl.batch_riemann_1d(8, seed=4096)]
+    probs += [(kw, q) for kw, q in [wl.riemann_1d(8192, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 0.5)]]
+    probs[-1][0].update(scheme="teno5", flux="hllc", cfl=0.5)
+    outs, _ = _run(probs, 60)
+    for (kw, q), out in zip(probs, outs):
+        ref, _ = _oracle(kw, q, 60)
+        assert_close(out, ref, what=f"riemann 1d n={kw['n'][0]} {kw['scheme']}/{kw['flux']}")
+
+
+def test_alone_equals_mixed_bitwise():
+    probs = _mixed_problems()
+    steps = 20
+    mixed, mclocks = _run(probs, steps)
+    for k in (0, 5, len(probs) - 3, len(probs) - 1):
+        alone, aclocks = _run([probs[k]], steps)
+        assert np.array_equal(alone[0], mixed[k]), f"problem {k} differs alone vs mixed"
+        assert aclocks[0] == mclocks[k]
+
+
+@pytest.mark.parametrize("graph,fused,tiles,occupancy", [
+    (False, True, (127, 32), 8), (True, False, (127, 32), 8), (True, True, (63, 8), 2),
+    (False, False, (255, 64), 1), (True, True, (255, 16), 32)])
+def test_schedule_and_shape_do_not_change_results(graph, fused, tiles, occupancy):
+    probs = _mixed_problems(count=6)
+    base, bclocks = _run(probs, 12)
+    other, oclocks = _run(probs, 12, graph=graph, fused=fused, tiles=tiles, occupancy=occupancy)
+    for k in range(len(probs)):
+        assert np.array_equal(base[k], other[k]), k
+        assert bclocks[k] == oclocks[k]
+
+
+def test_batch_matches_grid_path():
+    kw, q = wl.riemann_1d(500, (1.0, 0.75, 1.0), (0.125, 0.0, 0.1), 0.4)
+    kw.update(scheme="weno5", flux="hllc", cfl=0.5)
+    outs, clocks = _run([(kw, q)], 30)
+    with Grid(make_desc(n=(501, 1, 1), d=(1.0 / 500, 1.0, 1.0), scheme="weno5", flux="hllc",
+                        time="ssprk3")) as g:
+        g.upload(q)
+        dts = g.run(30, 0.5)
+        ref = g.download()
+    assert_close(outs[0], ref, what="batch vs grid")
+    assert abs(clocks[0][0] - dts.sum()) <= 1e-13 * dts.sum()
+
+
+def test_step_limits_freeze_problems():
+    probs = _mixed_problems(count=4)
+    cap = sum(item_nodes(make_item(**kw)) for kw, _ in probs) + 64
+    with Batch(cap, 16) as b:
+        slots = [b.add(make_item(**kw), q, max_steps=5 + 3 * k) for k, (kw, q) in enumerate(probs)]
+        b.step(40)
+        steps = b.steps()
+        for k, s in enumerate(slots):
+            assert steps[s] == 5 + 3 * k
+        outs = [b.download(s) for s in slots]
+    for k, ((kw, q), out) in enumerate(zip(probs, outs)):
+        ref, _ = _oracle(kw, q, 5 + 3 * k)
+        assert_close(out, ref, what=f"frozen problem {k}")
+
+
+def test_admission_by_footprint_and_reuse():
+    kw1, q1 = wl.riemann_1d(300, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 0.5)
+    kw1.update(scheme="weno5", flux="rusanov", cfl=0.5)
+    kw2, q2 = wl.quadrant_2d(40, QUADS)
+    kw2.update(scheme="weno5", flux="hllc", cfl=0.5)
+    n1, n2 = item_nodes(make_item(**kw1)), item_nodes(make_item(**kw2))
+    assert n1 == 307 and n2 == 46 * 46
+    with Batch(2 * n1 + n2, 8) as b:
+        a = b.add(make_item(**kw1), q1)
+        c = b.add(make_item(**kw2), q2)
+        a2 = b.add(make_item(**kw1), q1)
+        assert b.try_add(make_item(**kw1), q1) is None   # arena full
+        used, free, items = b.memory()
+        assert (used, free, items) == (2 * n1 + n2, 0, 3)
+        b.step(10)
+        b.remove(a)
+        b.remove(a2)
+        assert b.memory()[1] == 2 * n1
+        # a freed range is reused and the newcomer starts from step 0
+        a3 = b.add(make_item(**kw1), q1)
+        b.step(10)
+        t3, k3 = b.clock(a3)
+        tc, kc = b.clock(c)
+        assert (k3, kc) == (10, 20)
+        out1 = b.download(a3)
+        out2 = b.download(c)
+    ref1, _ = _oracle(kw1, q1, 10)
+    ref2, _ = _oracle(kw2, q2, 20)
+    assert_close(out1, ref1, what="readmitted 1d")
+    assert_close(out2, ref2, what="resident 2d")
