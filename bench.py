@@ -25,6 +25,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -63,6 +65,177 @@ def cpu_baseline(n: int, steps: int):
     }
 
 
+def _plain_case(kw, q, steps):
+    """One problem on its own grid through the single-case path (one launch
+    sequence per step); returns (state, seconds spent in the run)."""
+    from sonicsolver_amd import Grid, make_desc
+    n = kw["n"]
+    two_d = len(n) > 1 and n[1] > 1
+    desc = make_desc(n=(n[0], n[1] if two_d else 1, 1), active=(1, 1 if two_d else 0, 0),
+                     d=(kw["d"][0], kw["d"][1] if two_d else 1.0, 1.0), scheme=kw["scheme"],
+                     flux=kw["flux"], time="ssprk3")
+    with Grid(desc) as g:
+        g.upload(q)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.run(steps, kw["cfl"], record_dt=False)
+        g.synchronize()
+        el = time.perf_counter() - t0
+        return g.download(), el
+
+
+def _rel_err(a, b):
+    a = np.asarray(a).reshape(5, -1)
+    b = np.asarray(b).reshape(5, -1)
+    d = np.abs(a - b)
+    return float(max((d[v].sum() / max(np.abs(b[v]).sum(), 1e-300)) for v in range(5))), \
+        float(max((d[v].max() / max(np.abs(b[v]).max(), 1e-300)) for v in range(5)))
+
+
+def batched_run(count: int, steps: int, sample: int):
+    """Independent 1D Riemann problems of uneven length (256..8192 cells,
+    mixed WENO5/TENO5 and Rusanov/HLLC, CFL 0.5 each) advanced together by
+    the persistent batch kernels for `steps` steps each, against the same GPU
+    solving them one case at a time."""
+    from sonicsolver_amd import SonicError, workloads as wl
+    from sonicsolver_amd.batch import Batch, item_nodes, make_item
+    probs = [(kw, q) for _, kw, q in wl.batch_riemann_1d(count)]
+    cells = sum(kw["n"][0] - 1 for kw, _ in probs)
+    cap = sum(item_nodes(make_item(**kw)) for kw, _ in probs) + 64
+    picks = list(range(0, count, max(1, count // sample)))[:sample]
+    with Batch(cap, count) as b:
+        slots = [b.add(make_item(**kw), q, max_steps=steps) for kw, q in probs]
+        log(f"batched: {count} problems admitted ({cells} cells), autotuning")
+        tuning = b.autotune(3)
+        log(f"batched: tuned {tuning}; timing {steps} steps")
+        t0 = time.perf_counter()
+        try:
+            b.step(steps)
+        except SonicError as e:
+            if e.code != -3:
+                raise
+        elapsed = time.perf_counter() - t0
+        # a problem that reaches a non-physical state is frozen with its
+        # flags while the others run on (the reference's CPU path stops on
+        # the same problems)
+        faults = b.faults()[:count]
+        failed = [k for k in range(count) if faults[slots[k]]]
+        done = b.steps()[:count]
+        assert all(done[slots[k]] == steps for k in range(count) if not faults[slots[k]])
+        picks = [k for k in picks if not faults[slots[k]]]
+        batch_out = {k: b.download(slots[k]) for k in picks}
+    log(f"batched: {elapsed:.3f} s; plain single-case path on {len(picks)} of the problems")
+    plain_s = 0.0
+    plain_cells = 0
+    worst = (0.0, 0.0)
+    for k in picks:
+        kw, q = probs[k]
+        out, el = _plain_case(kw, q, steps)
+        plain_s += el
+        plain_cells += kw["n"][0] - 1
+        e = _rel_err(batch_out[k], out)
+        worst = (max(worst[0], e[0]), max(worst[1], e[1]))
+    solved = count - len(failed)
+    solved_cells = cells - sum(probs[k][0]["n"][0] - 1 for k in failed)
+    solves = solved / elapsed
+    plain_solves = len(picks) / plain_s
+    return {
+        "workload": f"{count} independent 1D Riemann problems, 256-8192 cells (log-uniform), "
+                    f"WENO5/TENO5 x Rusanov/HLLC, SSPRK3, per-problem CFL 0.5 dt, {steps} steps "
+                    "each, FP64",
+        "solves_per_s": round(solves, 2),
+        "mcell_updates_per_s": round(solved_cells * steps / elapsed / 1e6, 2),
+        "seconds": round(elapsed, 4),
+        "cells": cells,
+        "solved": solved,
+        "failed_nonphysical": len(failed),
+        "tuning": tuning,
+        "plain_single_case": {
+            "solves_per_s": round(plain_solves, 2),
+            "mcell_updates_per_s": round(plain_cells * steps / plain_s / 1e6, 2),
+            "sample": f"{len(picks)} of the problems, each on its own grid, sf_run per case",
+        },
+        "speedup_vs_plain": round(solves / plain_solves, 2),
+        "max_rel_diff_vs_plain": {"l1": worst[0], "linf": worst[1]},
+    }
+
+
+def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int = 5005):
+    """Served stream of four-quadrant 2D Riemann problems (N log-uniform in
+    [64, 1024], 50-500 steps, WENO5/HLLC/SSPRK3) arriving as a Poisson
+    process of `rate` requests/s; requests are admitted into the device arena
+    by footprint as memory frees up (continuous batching)."""
+    import threading
+    from sonicsolver_amd import workloads as wl
+    from sonicsolver_amd.batch import make_item
+    from sonicsolver_amd.serve import Server
+    reqs = [(kw, steps, make()) for _, kw, steps, make in wl.quadrant_requests(count, seed=seed)]
+    cells = sum(kw["n"][0] * kw["n"][1] * st for kw, st, _ in reqs)
+    gaps = -np.log(1.0 - wl.uniform01(seed + 1, count)) / rate
+    arrivals = np.cumsum(gaps) - gaps[0]
+    tickets = [None] * count
+    with Server(int(budget_gb * 2**30), max_items=1024, steps_per_tick=25) as srv:
+        # warm the kernels and the graph path on one small request
+        kw, st, q = reqs[0]
+        warm = srv.submit(make_item(**kw), q, 2)
+        srv.wait(warm, 60)
+        srv.result(warm)
+        srv.reset_stats()
+        log(f"served: {count} requests at {rate}/s (Poisson), arena {budget_gb} GiB")
+
+        def submitter():
+            t0 = time.perf_counter()
+            for k, (kw, st, q) in enumerate(reqs):
+                delay = arrivals[k] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[k] = srv.submit(make_item(**kw), q, st)
+
+        th = threading.Thread(target=submitter)
+        t0 = time.perf_counter()
+        th.start()
+        th.join()
+        srv.drain(600)
+        wall = time.perf_counter() - t0
+        stats = srv.stats()
+        outs = {}
+        picks = list(range(0, count, max(1, count // sample)))[:sample]
+        for k, t in enumerate(tickets):
+            out, tt, nst, lat = srv.result(t)
+            if k in picks:
+                outs[k] = out
+    plain_s = 0.0
+    worst = (0.0, 0.0)
+    for k in picks:
+        kw, st, q = reqs[k]
+        out, el = _plain_case(kw, q, st)
+        plain_s += el
+        e = _rel_err(outs[k], out)
+        worst = (max(worst[0], e[0]), max(worst[1], e[1]))
+    plain_solves = len(picks) / plain_s
+    return {
+        "workload": f"{count} four-quadrant 2D Riemann requests, N log-uniform [64,1024], "
+                    "50-500 steps, WENO5/HLLC/SSPRK3 FP64, Poisson arrivals",
+        "rate_per_s": rate,
+        "solves_per_s": round(stats["solves_per_s"], 2),
+        "p50_ms": round(stats["p50_ms"], 2),
+        "p99_ms": round(stats["p99_ms"], 2),
+        "mean_ms": round(stats["mean_ms"], 2),
+        "wall_s": round(wall, 3),
+        "mcell_updates_per_s": round(cells / wall / 1e6, 2),
+        "gpu_busy_frac": round(stats["busy_s"] / wall, 3),
+        "peak_resident": stats["peak_items"],
+        "completed": stats["completed"],
+        "failed": stats["failed"],
+        "plain_single_case": {
+            "solves_per_s": round(plain_solves, 2),
+            "sample": f"{len(picks)} of the requests, each on its own grid, back to back",
+        },
+        "speedup_vs_plain": round(stats["solves_per_s"] / plain_solves, 2),
+        "max_rel_diff_vs_plain": {"l1": worst[0], "linf": worst[1]},
+    }
+
+
 def traffic_measurement(kernel: str):
     """HBM bytes per launch of `kernel` measured with rocprofv3 PMC counters
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size) and
@@ -87,6 +260,15 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=96, help="n of the n^3 CPU sample")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched", type=int, default=4096,
+                    help="problems in the batched run (0 skips it)")
+    ap.add_argument("--batched-steps", type=int, default=300)
+    ap.add_argument("--served", type=int, default=240,
+                    help="requests in the served run (0 skips it)")
+    ap.add_argument("--served-rate", type=float, default=40.0, help="Poisson arrivals per s")
+    ap.add_argument("--served-budget-gb", type=float, default=24.0)
+    ap.add_argument("--plain-sample", type=int, default=32,
+                    help="problems solved one case at a time for the plain comparison")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -196,6 +378,11 @@ def main():
         "checks": {"device_flags": flags, "dt_first": dts[0] if len(dts) else None,
                    "mass_rel_drift": abs(mass1 - mass0) / abs(mass0)},
     }
+    if args.batched:
+        line["batched"] = batched_run(args.batched, args.batched_steps, args.plain_sample)
+    if args.served:
+        line["served"] = served_run(args.served, args.served_rate, args.served_budget_gb,
+                                    min(args.plain_sample, 24))
     if not args.no_cpu_baseline:
         log(f"CPU baseline: reference code on a {args.cpu_sample}^3 sample")
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cpu_steps)

@@ -233,6 +233,22 @@ int sf_batch_set_occupancy(sf_batch *batch, int blocks_per_cu);
  * launch per variant present (fewer registers per kernel).  Results do not
  * depend on it. */
 int sf_batch_set_schedule(sf_batch *batch, int fused);
+/* Per-problem device fault flags (SF_FLAG_*), item_capacity entries (0 for
+ * free slots).  A problem that raises a flag is frozen at that step; the
+ * others keep advancing, and sf_batch_step returns SF_E_NUMERIC. */
+int sf_batch_faults(sf_batch *batch, uint32_t *flags);
+
+/* Launch shape chosen by sf_batch_autotune. */
+typedef struct {
+    int xlen, ylen, blocks_per_cu, fused;
+    double ms_per_step;      /* mean step time of the chosen shape            */
+    int candidates;          /* shapes timed                                  */
+} sf_batch_tuning;
+/* Time every launch shape (schedule, x tile, y tile, persistent grid size)
+ * on the admitted problems for trial_steps steps each, from a saved state
+ * that is restored after every trial, and keep the fastest.  The problems'
+ * states and clocks are unchanged on return. */
+int sf_batch_autotune(sf_batch *batch, int trial_steps, sf_batch_tuning *out);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
 REF_SO = HERE / "_ref" / "libsfref.so"
+REF_FMA_SO = HERE / "_ref" / "libsfref_fma.so"
 
 _P = ctypes.POINTER(ctypes.c_double)
 _U = ctypes.POINTER(ctypes.c_uint)
@@ -99,25 +100,43 @@ def ref_available() -> bool:
     return REF_SO.exists()
 
 
-def ref():
-    global _ref
+def ref_fma_available() -> bool:
+    return REF_FMA_SO.exists()
+
+
+_ref_fma = None
+
+
+def ref(fma: bool = False):
+    """The reference harness; fma=True loads the build with FMA contraction
+    (the reference's release flags)."""
+    global _ref, _ref_fma
+    if fma:
+        if _ref_fma is None:
+            if not REF_FMA_SO.exists():
+                raise FileNotFoundError(f"{REF_FMA_SO} has not been built (make -C oracle ref_fma)")
+            _ref_fma = _declare_ref(ctypes.CDLL(str(REF_FMA_SO)))
+        return _ref_fma
     if _ref is None:
         if not REF_SO.exists():
             raise FileNotFoundError(f"{REF_SO} has not been built (make -C oracle ref)")
-        lib = ctypes.CDLL(str(REF_SO))
-        lib.sfref_run.argtypes = [ctypes.POINTER(Desc), _P, ctypes.c_int, ctypes.c_double,
-                                  ctypes.c_double, _P]
-        lib.sfref_residual.argtypes = [ctypes.POINTER(Desc), _P, _P]
-        lib.sfref_weno5.argtypes = [_P]
-        lib.sfref_weno5.restype = ctypes.c_double
-        lib.sfref_teno5.argtypes = [_P]
-        lib.sfref_teno5.restype = ctypes.c_double
-        lib.sfref_char_matrix.argtypes = [_P, _P, ctypes.c_double, _P, _P]
-        lib.sfref_rusanov.argtypes = [_P, _P, _P, ctypes.c_double, _P]
-        lib.sfref_char_face_flux.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int,
-                                             ctypes.c_double, _P]
-        _ref = lib
+        _ref = _declare_ref(ctypes.CDLL(str(REF_SO)))
     return _ref
+
+
+def _declare_ref(lib):
+    lib.sfref_run.argtypes = [ctypes.POINTER(Desc), _P, ctypes.c_int, ctypes.c_double,
+                              ctypes.c_double, _P]
+    lib.sfref_residual.argtypes = [ctypes.POINTER(Desc), _P, _P]
+    lib.sfref_weno5.argtypes = [_P]
+    lib.sfref_weno5.restype = ctypes.c_double
+    lib.sfref_teno5.argtypes = [_P]
+    lib.sfref_teno5.restype = ctypes.c_double
+    lib.sfref_char_matrix.argtypes = [_P, _P, ctypes.c_double, _P, _P]
+    lib.sfref_rusanov.argtypes = [_P, _P, _P, ctypes.c_double, _P]
+    lib.sfref_char_face_flux.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, _P]
+    return lib
 
 
 def _arr(x):
@@ -204,11 +223,13 @@ def compute_dt(desc: Desc, q: np.ndarray, cfl: float) -> float:
         lib.sfo_grid_destroy(g)
 
 
-def ref_run(desc: Desc, q: np.ndarray, nsteps: int, cfl: float, max_dt: float = 1e300):
-    """Run the reference's own code through the harness."""
+def ref_run(desc: Desc, q: np.ndarray, nsteps: int, cfl: float, max_dt: float = 1e300,
+            fma: bool = False):
+    """Run the reference's own code through the harness (fma=True: the
+    build with FMA contraction)."""
     a = _arr(q).reshape(5, -1).copy()
     dts = np.zeros(max(nsteps, 1))
-    ref().sfref_run(ctypes.byref(desc), _ptr(a), nsteps, cfl, max_dt, _ptr(dts))
+    ref(fma).sfref_run(ctypes.byref(desc), _ptr(a), nsteps, cfl, max_dt, _ptr(dts))
     return a, dts[:nsteps]
 
 

@@ -20,6 +20,12 @@ _VP = ctypes.c_void_p
 BYTES_PER_NODE = 4 * 5 * 8  # three state buffers + flux accumulator, 5 FP64 each
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("xlen", ctypes.c_int), ("ylen", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
+                ("fused", ctypes.c_int), ("ms_per_step", ctypes.c_double),
+                ("candidates", ctypes.c_int)]
+
+
 class BatchItem(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int * 2), ("d", ctypes.c_double * 2), ("scheme", ctypes.c_int),
                 ("flux", ctypes.c_int), ("cfl", ctypes.c_double)]
@@ -61,6 +67,8 @@ def _lib():
             "sf_batch_set_tiles": [_VP, ctypes.c_int, ctypes.c_int],
             "sf_batch_set_occupancy": [_VP, ctypes.c_int],
             "sf_batch_set_schedule": [_VP, ctypes.c_int],
+            "sf_batch_faults": [_VP, ctypes.POINTER(ctypes.c_uint32)],
+            "sf_batch_autotune": [_VP, ctypes.c_int, ctypes.POINTER(Tuning)],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)
@@ -171,3 +179,17 @@ class Batch:
 
     def set_schedule(self, fused: bool):
         _check(self._lib.sf_batch_set_schedule(self._h, 1 if fused else 0))
+
+    def faults(self) -> np.ndarray:
+        """Per-slot device fault flags (0 for healthy or free slots)."""
+        out = np.zeros(self.item_capacity, dtype=np.uint32)
+        _check(self._lib.sf_batch_faults(self._h, out.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_uint32))))
+        return out
+
+    def autotune(self, trial_steps: int = 3) -> dict:
+        """Time every launch shape on the admitted problems and keep the
+        fastest; the problems' states and clocks are left unchanged."""
+        t = Tuning()
+        _check(self._lib.sf_batch_autotune(self._h, int(trial_steps), ctypes.byref(t)))
+        return {name: getattr(t, name) for name, _ in t._fields_}

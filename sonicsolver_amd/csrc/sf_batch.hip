@@ -53,8 +53,8 @@ struct BatchDev {
     unsigned long long* dtmin;
     double* time;
     long long* steps;
-    const long long* target;  // a problem is live while steps < target
-    unsigned* flags;
+    const long long* target;  // a problem is live while steps < target ...
+    unsigned* flags;          // ... and raised no fault flags (per problem)
 };
 
 struct BatchStage {
@@ -88,7 +88,7 @@ __device__ __forceinline__ ItemView item_view(const BatchDev& B, const BatchItem
 }
 
 __device__ __forceinline__ bool item_live(const BatchDev& B, int slot) {
-    return B.steps[slot] < B.target[slot];
+    return B.steps[slot] < B.target[slot] && B.flags[slot] == 0;
 }
 
 __device__ __forceinline__ long long item_index(const BatchItemDev& it, int i, int j) {
@@ -148,7 +148,7 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
             A.q0 = st.q0;
             A.qout = st.qout;
             A.dt = &B.dt[T.x];
-            A.flags = B.flags;
+            A.flags = &B.flags[T.x];
             A.base = st.base;
             A.incr = st.incr;
             A.cfl = it.cfl;
@@ -160,9 +160,12 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
             else carry_node(G, A, id, q);
         }
         if (st.reduce_dt) block_min_atomic(dl, &B.dtmin[T.x], red);
+        if (flags) {
+            atomicOr(&B.flags[T.x], flags);
+            flags = 0;
+        }
         __syncthreads();
     }
-    if (flags) atomicOr(B.flags, flags);
 }
 
 // Tile ranges and block ranges of the four (scheme, flux) variants,
@@ -220,23 +223,22 @@ __device__ __forceinline__ void y_tiles(const BatchDev& B, const double* __restr
         const int a1 = min(a0 + B.yseg, it.n1);
         if (i >= it.n0) continue;  // per lane; no barrier below
         const long long step = it.m0;
+        // nodes a0-3 .. a0+1 into slots 1..5; the march starts at a0-1 so
+        // every face is evaluated by the same instructions (see
+        // k_sweep_march)
         long long id = item_index(it, i, a0) - 3 * step;
-        for (int s = 0; s < 6; ++s)
+        for (int s = 0; s < 5; ++s)
 #pragma unroll
-            for (int v = 0; v < 5; ++v) win[s][v][lane] = q[v * B.P + id + s * step];
-        double stc[6][5];
-#pragma unroll
-        for (int s = 0; s < 6; ++s)
-#pragma unroll
-            for (int v = 0; v < 5; ++v) stc[s][v] = win[s][v][lane];
-        double fprev[5];
-        face_flux<1, SCHEME, FLUX>(stc, B.gamma, it.area1, fprev, flags);
-        id += 3 * step;
+            for (int v = 0; v < 5; ++v) win[s + 1][v][lane] = q[v * B.P + id + s * step];
+        id += 2 * step;
         int head = 0;
-        for (int a = a0; a < a1; ++a, id += step) {
+        double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma clang loop unroll(disable)
+        for (int a = a0 - 1; a < a1; ++a, id += step) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) win[head][v][lane] = q[v * B.P + id + 3 * step];
             head = (head == 5) ? 0 : head + 1;
+            double stc[6][5];
 #pragma unroll
             for (int s = 0; s < 6; ++s) {
                 const int slot = (head + s >= 6) ? head + s - 6 : head + s;
@@ -245,15 +247,22 @@ __device__ __forceinline__ void y_tiles(const BatchDev& B, const double* __restr
             }
             double f[5];
             face_flux<1, SCHEME, FLUX>(stc, B.gamma, it.area1, f, flags);
+            double d[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 if (!isfinite(f[v])) flags |= kFlagFlux;
-                S[v * B.P + id] = f[v] - fprev[v];
+                d[v] = f[v] - fprev[v];
                 fprev[v] = f[v];
             }
+            if (a < a0) continue;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) S[v * B.P + id] = d[v];
+        }
+        if (flags) {
+            atomicOr(&B.flags[T.x], flags);
+            flags = 0;
         }
     }
-    if (flags) atomicOr(B.flags, flags);
 }
 
 __global__ __launch_bounds__(64) void k_batch_y_fused(BatchDev B, const double* __restrict__ q,
@@ -311,7 +320,7 @@ __global__ __launch_bounds__(256) void k_batch_fill(BatchDev B, double* __restri
             const double ps = pressure_of(q, B.P, s, B.gamma);
             q[4 * B.P + b] = energy_from_pressure(q, B.P, b, B.gamma, ps);
             const double pb = pressure_of(q, B.P, b, B.gamma);
-            if (!(ps > 0.0) || !(pb > 0.0)) atomicOr(B.flags, (unsigned)kFlagState);
+            if (!(ps > 0.0) || !(pb > 0.0)) atomicOr(&B.flags[slot], (unsigned)kFlagState);
 #pragma unroll
             for (int l = 1; l <= 3; ++l)
                 q[4 * B.P + b + l * dir] = energy_from_pressure(q, B.P, b + l * dir, B.gamma, pb);
@@ -341,7 +350,7 @@ __global__ __launch_bounds__(256) void k_batch_cfl(BatchDev B, const int* __rest
         for (int v = 0; v < 5; ++v) qq[v] = q[v * B.P + id];
         best = fmin(best, local_dt(G, qq, it.cfl, flags));
     }
-    if (flags) atomicOr(B.flags, flags);
+    if (flags) atomicOr(&B.flags[slot], flags);
     block_min_atomic(best, &B.dtmin[slot], red);
 }
 
@@ -373,6 +382,7 @@ __global__ void k_batch_reset(BatchDev B, const int* __restrict__ list, int n) {
     B.time[slot] = 0.0;
     B.steps[slot] = 0;
     B.dt[slot] = 0.0;
+    B.flags[slot] = 0;
 }
 
 // unique-node SoA of one problem <-> its padded block in the arena
@@ -706,6 +716,39 @@ int prepare_pending(sf_batch* b) {
     return 0;
 }
 
+// Queue nsteps steps (graph replay or direct launches); no wait.
+int advance(sf_batch* b, int nsteps, int use_graph) {
+    if (use_graph) {
+        if (!b->graph_exec) {
+            SF_BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(b);
+            hipGraph_t g = nullptr;
+            SF_BHIP(hipStreamEndCapture(b->stream, &g));
+            b->graph = g;
+            SF_BHIP(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
+        }
+        for (int s = 0; s < nsteps; ++s) SF_BHIP(hipGraphLaunch(b->graph_exec, b->stream));
+    } else {
+        for (int s = 0; s < nsteps; ++s) enqueue_step(b);
+    }
+    SF_BHIP(hipGetLastError());
+    return 0;
+}
+
+// Per-problem fault flags of the admitted problems (0 for free slots);
+// returns the number of faulted problems.
+int read_faults(sf_batch* b, unsigned* flags, int* faulted) {
+    SF_BHIP(hipMemcpy(flags, b->d_flags, sizeof(unsigned) * b->item_capacity,
+                      hipMemcpyDeviceToHost));
+    int n = 0;
+    for (int s = 0; s < b->item_capacity; ++s) {
+        if (!b->used[s]) flags[s] = 0;
+        n += flags[s] != 0;
+    }
+    *faulted = n;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -745,11 +788,11 @@ int sf_batch_create(int device, double gamma, size_t node_capacity, int item_cap
     ok = ok && hipMalloc(&b->d_time, sizeof(double) * item_capacity) == hipSuccess;
     ok = ok && hipMalloc(&b->d_steps, sizeof(long long) * item_capacity) == hipSuccess;
     ok = ok && hipMalloc(&b->d_target, sizeof(long long) * item_capacity) == hipSuccess;
-    ok = ok && hipMalloc(&b->d_flags, sizeof(unsigned)) == hipSuccess;
+    ok = ok && hipMalloc(&b->d_flags, sizeof(unsigned) * item_capacity) == hipSuccess;
     ok = ok && hipMalloc(&b->d_active, sizeof(int) * item_capacity) == hipSuccess;
     ok = ok && hipMalloc(&b->d_scan, sizeof(int) * item_capacity) == hipSuccess;
     ok = ok && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
-    ok = ok && hipMemset(b->d_flags, 0, sizeof(unsigned)) == hipSuccess;
+    ok = ok && hipMemset(b->d_flags, 0, sizeof(unsigned) * item_capacity) == hipSuccess;
     ok = ok && hipMemset(b->d_steps, 0, sizeof(long long) * item_capacity) == hipSuccess;
     ok = ok && hipMemset(b->d_target, 0, sizeof(long long) * item_capacity) == hipSuccess;
     // the arena starts zeroed, so padding that is never written stays finite
@@ -908,29 +951,30 @@ int sf_batch_step(sf_batch* b, int nsteps, int use_graph) {
         if (int rc = rebuild_tables(b)) return rc;
     if (int rc = prepare_pending(b)) return rc;
     if (b->nactive == 0 || nsteps == 0) return 0;
-    if (use_graph) {
-        if (!b->graph_exec) {
-            SF_BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(b);
-            hipGraph_t g = nullptr;
-            SF_BHIP(hipStreamEndCapture(b->stream, &g));
-            b->graph = g;
-            SF_BHIP(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
-        }
-        for (int s = 0; s < nsteps; ++s) SF_BHIP(hipGraphLaunch(b->graph_exec, b->stream));
-    } else {
-        for (int s = 0; s < nsteps; ++s) enqueue_step(b);
-    }
-    SF_BHIP(hipGetLastError());
+    if (int rc = advance(b, nsteps, use_graph)) return rc;
     SF_BHIP(hipStreamSynchronize(b->stream));
-    unsigned h = 0;
-    SF_BHIP(hipMemcpy(&h, b->d_flags, sizeof h, hipMemcpyDeviceToHost));
-    if (h) {
-        char msg[160];
-        std::snprintf(msg, sizeof msg, "sf_batch_step: device raised numerical fault flags 0x%x", h);
+    std::vector<unsigned> flags((size_t)b->item_capacity);
+    int faulted = 0;
+    if (int rc = read_faults(b, flags.data(), &faulted)) return rc;
+    if (faulted) {
+        int first = 0;
+        while (!flags[first]) ++first;
+        char msg[200];
+        std::snprintf(msg, sizeof msg,
+                      "sf_batch_step: %d problem(s) raised numerical fault flags (slot %d: 0x%x); "
+                      "they are frozen, the others advanced",
+                      faulted, first, flags[first]);
         return bfail(SF_E_NUMERIC, msg);
     }
     return 0;
+}
+
+int sf_batch_faults(sf_batch* b, uint32_t* flags) {
+    if (!b || !flags) return bfail(SF_E_ARG, "sf_batch_faults: null argument");
+    SF_BHIP(hipSetDevice(b->device));
+    SF_BHIP(hipStreamSynchronize(b->stream));
+    int faulted = 0;
+    return read_faults(b, flags, &faulted);
 }
 
 int sf_batch_clock(sf_batch* b, int slot, double* time, long long* steps) {
@@ -976,6 +1020,144 @@ int sf_batch_set_tiles(sf_batch* b, int xlen, int ylen) {
 }
 
 }  // extern "C"
+
+// ---- autotuner ----------------------------------------------------------------
+// Times every launch shape (schedule x x-tile x y-tile x persistent grid
+// size) on the batch's own problems: each trial replays the captured step
+// from the same saved state, which is restored afterwards, so tuning leaves
+// the problems exactly as it found them.
+namespace {
+
+struct Snapshot {
+    double* q = nullptr;
+    long long extent = 0;
+    std::vector<double> dt, time;
+    std::vector<unsigned long long> dtmin;
+    std::vector<long long> steps;
+    std::vector<unsigned> flags;
+};
+
+int snapshot_save(sf_batch* b, Snapshot& sn) {
+    sn.extent = 0;
+    for (int s = 0; s < b->item_capacity; ++s)
+        if (b->used[s])
+            sn.extent = std::max(sn.extent, b->items[s].off + padded_nodes(b->desc[s]));
+    SF_BHIP(hipMalloc(&sn.q, sizeof(double) * 5 * (size_t)std::max(1LL, sn.extent)));
+    for (int v = 0; v < 5; ++v)
+        SF_BHIP(hipMemcpyAsync(sn.q + v * sn.extent, b->buf[0] + v * b->capacity,
+                               sizeof(double) * sn.extent, hipMemcpyDeviceToDevice, b->stream));
+    const size_t n = (size_t)b->item_capacity;
+    sn.dt.resize(n);
+    sn.time.resize(n);
+    sn.dtmin.resize(n);
+    sn.steps.resize(n);
+    sn.flags.resize(n);
+    SF_BHIP(hipStreamSynchronize(b->stream));
+    SF_BHIP(hipMemcpy(sn.dt.data(), b->d_dt, sizeof(double) * n, hipMemcpyDeviceToHost));
+    SF_BHIP(hipMemcpy(sn.time.data(), b->d_time, sizeof(double) * n, hipMemcpyDeviceToHost));
+    SF_BHIP(hipMemcpy(sn.dtmin.data(), b->d_dtmin, 8 * n, hipMemcpyDeviceToHost));
+    SF_BHIP(hipMemcpy(sn.steps.data(), b->d_steps, 8 * n, hipMemcpyDeviceToHost));
+    SF_BHIP(hipMemcpy(sn.flags.data(), b->d_flags, 4 * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int snapshot_restore(sf_batch* b, const Snapshot& sn) {
+    for (int v = 0; v < 5; ++v)
+        SF_BHIP(hipMemcpyAsync(b->buf[0] + v * b->capacity, sn.q + v * sn.extent,
+                               sizeof(double) * sn.extent, hipMemcpyDeviceToDevice, b->stream));
+    SF_BHIP(hipStreamSynchronize(b->stream));
+    const size_t n = (size_t)b->item_capacity;
+    SF_BHIP(hipMemcpy(b->d_dt, sn.dt.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    SF_BHIP(hipMemcpy(b->d_time, sn.time.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    SF_BHIP(hipMemcpy(b->d_dtmin, sn.dtmin.data(), 8 * n, hipMemcpyHostToDevice));
+    SF_BHIP(hipMemcpy(b->d_steps, sn.steps.data(), 8 * n, hipMemcpyHostToDevice));
+    SF_BHIP(hipMemcpy(b->d_flags, sn.flags.data(), 4 * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Mean time of one step of the current shape over `steps` graph replays
+// (after one untimed replay that captures the graph).
+int time_shape(sf_batch* b, int steps, double* ms) {
+    if (b->tables_dirty)
+        if (int rc = rebuild_tables(b)) return rc;
+    if (int rc = advance(b, 1, 1)) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SF_BHIP(hipEventCreate(&e0));
+    SF_BHIP(hipEventCreate(&e1));
+    int rc = 0;
+    if (hipEventRecord(e0, b->stream) != hipSuccess) rc = bfail(SF_E_HIP, "event record failed");
+    if (!rc) rc = advance(b, steps, 1);
+    if (!rc && hipEventRecord(e1, b->stream) != hipSuccess)
+        rc = bfail(SF_E_HIP, "event record failed");
+    if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = bfail(SF_E_HIP, "event wait failed");
+    float t = 0.0f;
+    if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+        rc = bfail(SF_E_HIP, "event timing failed");
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = (double)t / steps;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int sf_batch_autotune(sf_batch* b, int trial_steps, sf_batch_tuning* out) {
+    if (!b || !out || trial_steps < 1) return bfail(SF_E_ARG, "sf_batch_autotune: invalid argument");
+    SF_BHIP(hipSetDevice(b->device));
+    if (b->tables_dirty)
+        if (int rc = rebuild_tables(b)) return rc;
+    if (int rc = prepare_pending(b)) return rc;
+    if (b->nactive == 0) return bfail(SF_E_STATE, "sf_batch_autotune: the batch is empty");
+    Snapshot sn;
+    int rc = snapshot_save(b, sn);
+    const int keep[4] = {b->xbs, b->yseg, b->blocks_per_cu, b->fused};
+    sf_batch_tuning best{};
+    best.ms_per_step = 1e300;
+    int tried = 0;
+    const int xlens[3] = {63, 127, 255};
+    const int occs[4] = {2, 4, 8, 16};
+    std::vector<int> ysegs = b->any2d ? std::vector<int>{16, 32, 64} : std::vector<int>{b->yseg};
+    for (int fused = 0; fused < 2 && !rc; ++fused)
+        for (int xl : xlens)
+            for (int ys : ysegs)
+                for (int oc : occs) {
+                    if (rc) break;
+                    b->fused = fused;
+                    b->xbs = xl + 1;
+                    b->yseg = ys;
+                    b->blocks_per_cu = oc;
+                    b->tables_dirty = true;
+                    double ms = 0.0;
+                    rc = time_shape(b, trial_steps, &ms);
+                    if (!rc) rc = snapshot_restore(b, sn);
+                    if (rc) break;
+                    ++tried;
+                    if (ms < best.ms_per_step) {
+                        best.xlen = xl;
+                        best.ylen = ys;
+                        best.blocks_per_cu = oc;
+                        best.fused = fused;
+                        best.ms_per_step = ms;
+                    }
+                }
+    if (sn.q) hipFree(sn.q);
+    if (rc) {
+        b->xbs = keep[0];
+        b->yseg = keep[1];
+        b->blocks_per_cu = keep[2];
+        b->fused = keep[3];
+        b->tables_dirty = true;
+        return rc;
+    }
+    best.candidates = tried;
+    b->xbs = best.xlen + 1;
+    b->yseg = best.ylen;
+    b->blocks_per_cu = best.blocks_per_cu;
+    b->fused = best.fused;
+    b->tables_dirty = true;
+    *out = best;
+    return 0;
+}
 
 extern "C" int sf_batch_set_schedule(sf_batch* b, int fused) {
     if (!b) return bfail(SF_E_ARG, "sf_batch_set_schedule: null batch");

@@ -69,12 +69,38 @@ __device__ __forceinline__ double weno5(double v0, double v1, double v2,
     return (w0 * q0 + w1 * q1 + w2 * q2) / (6.0 * (w0 + w1 + w2));
 }
 
+// x^6 to within a fraction of an ulp beyond the final rounding: the
+// products are carried with their exact rounding errors (FMA two-product),
+// so the result is the correctly rounded x^6 except within ~2^-100 of a
+// rounding boundary -- the value std::pow(x, 6.0) returns in round-to-nearest
+// except in its own rare last-ulp misses.
+__device__ __forceinline__ double pow6(double x) {
+#pragma clang fp contract(off)
+    const double p2 = x * x;
+    const double e2 = __builtin_fma(x, x, -p2);
+    const double p3 = p2 * x;
+    const double e3 = __builtin_fma(p2, x, -p3) + e2 * x;
+    const double p6 = p3 * p3;
+    const double e6 = __builtin_fma(p3, p3, -p6) + 2.0 * p3 * e3;
+    return p6 + e6;
+}
+
+// The reference's cutoff test RN(g / gs) < CT (CT = 1e-5) without the
+// division.  RN(x) < CT exactly when x < CT - d with d half the gap from CT
+// down to the next double (2^-70 for CT = 1e-5), i.e. when
+// g < CT*gs - d*gs.  CT*gs = P + E exactly (FMA), d*gs is exact, and near
+// the boundary g - P is exact (Sterbenz), so the comparison below decides
+// exactly as the division does (up to a tie at the midpoint).
+__device__ __forceinline__ bool teno_cut(double g, double gs) {
+#pragma clang fp contract(off)
+    const double P = 1.0e-5 * gs;
+    const double E = __builtin_fma(1.0e-5, gs, -P);
+    return g - P < E - 0x1p-70 * gs;
+}
+
 // TENO5::teno5_core, src/methods/numerics/convection/SF_TENO5.cpp:44-92
-// (C = 1, q = 6, CT = 1e-5).  gamma^6 is formed by three products instead
-// of std::pow, and the cutoff test gamma_k / sum < CT is evaluated as
-// gamma_k < CT * sum; both agree with the reference to a few ulps, which
-// can only matter where gamma_k / sum lands within that distance of CT.
-// The 1/6 of the candidates is folded into the final division.
+// (C = 1, q = 6, CT = 1e-5).  The 1/6 of the candidates is folded into the
+// final division.
 __device__ __forceinline__ double teno5(double v0, double v1, double v2,
                                         double v3, double v4, unsigned& flags) {
     const double q0 = 2.0 * v0 - 7.0 * v1 + 11.0 * v2;
@@ -87,16 +113,13 @@ __device__ __forceinline__ double teno5(double v0, double v1, double v2,
     const double b2 = 13.0 / 12.0 * sq(v2 - 2.0 * v3 + v4)
                     + 0.25 * sq(3.0 * v2 - 4.0 * v3 + v4);
     const double tau5 = fabs(b0 - b2);
-    double g0 = 1.0 + tau5 / (b0 + 1.0e-40);
-    double g1 = 1.0 + tau5 / (b1 + 1.0e-40);
-    double g2 = 1.0 + tau5 / (b2 + 1.0e-40);
-    g0 = g0 * g0 * g0; g0 = g0 * g0;
-    g1 = g1 * g1 * g1; g1 = g1 * g1;
-    g2 = g2 * g2 * g2; g2 = g2 * g2;
-    const double cut = 1.0e-5 * (g0 + g1 + g2);
-    const double c0 = (g0 < cut) ? 0.0 : 1.0;
-    const double c1 = (g1 < cut) ? 0.0 : 1.0;
-    const double c2 = (g2 < cut) ? 0.0 : 1.0;
+    const double g0 = pow6(1.0 + tau5 / (b0 + 1.0e-40));
+    const double g1 = pow6(1.0 + tau5 / (b1 + 1.0e-40));
+    const double g2 = pow6(1.0 + tau5 / (b2 + 1.0e-40));
+    const double gs = g0 + g1 + g2;
+    const double c0 = teno_cut(g0, gs) ? 0.0 : 1.0;
+    const double c1 = teno_cut(g1, gs) ? 0.0 : 1.0;
+    const double c2 = teno_cut(g2, gs) ? 0.0 : 1.0;
     const double active = 0.1 * c0 + 0.6 * c1 + 0.3 * c2;
     if (!(active > 0.0)) flags |= kFlagTeno;
     return (0.1 * c0 * q0 + 0.6 * c1 * q1 + 0.3 * c2 * q2) / (6.0 * active);

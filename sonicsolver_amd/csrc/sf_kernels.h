@@ -114,25 +114,24 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
         const int j0 = (AXIS == 1) ? a0 : o;
         const int k0 = (AXIS == 2) ? a0 : o;
         const long long step = G.stride[AXIS];
-        // node (a0 - 3) along the axis
+        // Preload nodes a0-3 .. a0+1 into slots 1..5; the march starts one
+        // node early (a = a0-1) so that face a0-1/2 is evaluated by the same
+        // instructions as every other face: a face's rounding then does not
+        // depend on where a segment starts.
         long long id = node_index(G, i, j0, k0) - 3 * step;
-        for (int s = 0; s < 6; ++s)
+        for (int s = 0; s < 5; ++s)
 #pragma unroll
-            for (int v = 0; v < 5; ++v) win[s][v][lane] = q[v * G.P + id + s * step];
-        double st[6][5];
-#pragma unroll
-        for (int s = 0; s < 6; ++s)
-#pragma unroll
-            for (int v = 0; v < 5; ++v) st[s][v] = win[s][v][lane];
-        double fprev[5];
-        face_flux<AXIS, SCHEME, FLUX>(st, G.gamma, G.area[AXIS], fprev, flags);
-        id += 3 * step;  // node a0
-        int head = 0;    // slot holding stencil position 0
-        for (int a = a0; a < a1; ++a, id += step) {
+            for (int v = 0; v < 5; ++v) win[s + 1][v][lane] = q[v * G.P + id + s * step];
+        id += 2 * step;  // node a0 - 1
+        int head = 0;    // slot that receives the next node
+        double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma clang loop unroll(disable)
+        for (int a = a0 - 1; a < a1; ++a, id += step) {
             // drop the oldest node, append node a+3
 #pragma unroll
             for (int v = 0; v < 5; ++v) win[head][v][lane] = q[v * G.P + id + 3 * step];
             head = (head == 5) ? 0 : head + 1;
+            double st[6][5];
 #pragma unroll
             for (int s = 0; s < 6; ++s) {
                 const int slot = (head + s >= 6) ? head + s - 6 : head + s;
@@ -145,10 +144,13 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 if (!isfinite(f[v])) flags |= kFlagFlux;
-                const double d = f[v] - fprev[v];
-                s5[v] = (MODE == kCopyS) ? d : S[v * G.P + id] + d;
+                s5[v] = f[v] - fprev[v];
                 fprev[v] = f[v];
             }
+            if (a < a0) continue;
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                if (MODE != kCopyS) s5[v] = S[v * G.P + id] + s5[v];
             if (MODE == kFinal) {
                 const int jj = (AXIS == 1) ? a : o;
                 const int kk = (AXIS == 2) ? a : o;

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -191,7 +192,28 @@ void loop(sf_server* s) {
             s->busy_s += dt;
             ++s->ticks;
         }
-        if (rc != 0) {
+        if (rc == SF_E_NUMERIC) {
+            // only the problems that raised flags are failed; they are
+            // frozen on the device and the others advanced normally
+            const std::string why = last_error();
+            std::vector<uint32_t> flags((size_t)s->max_items);
+            if (sf_batch_faults(s->batch, flags.data()) != 0) {
+                drop_running(s, "batch step failed: " + why);
+                continue;
+            }
+            for (auto it = s->running.begin(); it != s->running.end();) {
+                const RequestPtr r = *it;
+                if (!flags[(size_t)r->slot]) {
+                    ++it;
+                    continue;
+                }
+                char msg[96];
+                std::snprintf(msg, sizeof msg, "numerical fault flags 0x%x", flags[(size_t)r->slot]);
+                sf_batch_remove(s->batch, r->slot);
+                it = s->running.erase(it);
+                finish(s, r, -1, msg);
+            }
+        } else if (rc != 0) {
             drop_running(s, "batch step failed: " + last_error());
             continue;
         }

@@ -141,3 +141,28 @@ def test_time_driver_bit_exact(end, end_step, by_step, wsteps, wtime):
                           write_by_step=bool(by_step), write_steps=wsteps, write_time=wtime)
     assert n.value == len(bd)
     assert np.array_equal(d[:n.value], bd) and np.array_equal(a, b)
+
+
+@pytest.mark.skipif(not ob.ref_fma_available(), reason="reference FMA build not built")
+def test_teno5_last_bit_sensitivity_of_the_reference():
+    """How far the reference moves from itself when only FMA contraction
+    changes (its release build contracts; the harness build does not), on
+    the batched configuration's Riemann problems.  WENO5 stays at rounding
+    level after 60 steps; TENO5 (cutoff 1e-5, epsilon 1e-40) stays within
+    1e-12 for 20 steps and then amplifies the last-bit differences far past
+    1e-11.  This is the yardstick the device TENO5 long-horizon parity is
+    measured against."""
+    from tests.parity import rel_errors
+    worst = {"weno5": 0.0, "teno5": 0.0}
+    for _, kw, q in wl.batch_riemann_1d(8, seed=4096):
+        n = kw["n"][0]
+        d = ob.make_desc(n=(n, 1, 1), active=(1, 0, 0), d=(kw["d"][0], 1.0, 1.0),
+                         scheme=kw["scheme"], flux=kw["flux"], time="ssprk3")
+        a, _ = ob.ref_run(d, q, 20, 0.5)
+        b, _ = ob.ref_run(d, q, 20, 0.5, fma=True)
+        assert rel_errors(b, a)[1].max() <= 1e-12
+        a, _ = ob.ref_run(d, q, 60, 0.5)
+        b, _ = ob.ref_run(d, q, 60, 0.5, fma=True)
+        worst[kw["scheme"]] = max(worst[kw["scheme"]], rel_errors(b, a)[1].max())
+    assert worst["weno5"] <= 1e-13
+    assert worst["teno5"] > 1e-9
