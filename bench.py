@@ -160,28 +160,26 @@ def batched_run(count: int, steps: int, sample: int):
     }
 
 
-def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int = 5005):
-    """Served stream of four-quadrant 2D Riemann problems (N log-uniform in
-    [64, 1024], 50-500 steps, WENO5/HLLC/SSPRK3) arriving as a Poisson
-    process of `rate` requests/s; requests are admitted into the device arena
-    by footprint as memory frees up (continuous batching)."""
+def _serve_pass(reqs, rate, budget_gb, seed, keep):
+    """One pass of the request stream through a fresh server: Poisson
+    arrivals at `rate`/s submitted from a client thread; returns the
+    server's statistics, the wall time and the results of the `keep`
+    requests."""
     import threading
-    from sonicsolver_amd import workloads as wl
+    from sonicsolver_amd import SonicError, workloads as wl
     from sonicsolver_amd.batch import make_item
     from sonicsolver_amd.serve import Server
-    reqs = [(kw, steps, make()) for _, kw, steps, make in wl.quadrant_requests(count, seed=seed)]
-    cells = sum(kw["n"][0] * kw["n"][1] * st for kw, st, _ in reqs)
-    gaps = -np.log(1.0 - wl.uniform01(seed + 1, count)) / rate
+    count = len(reqs)
+    gaps = -np.log(1.0 - wl.uniform01(seed, count)) / rate
     arrivals = np.cumsum(gaps) - gaps[0]
     tickets = [None] * count
     with Server(int(budget_gb * 2**30), max_items=1024, steps_per_tick=25) as srv:
         # warm the kernels and the graph path on one small request
-        kw, st, q = reqs[0]
+        kw, st, q = min(reqs, key=lambda r: r[0]["n"][0])
         warm = srv.submit(make_item(**kw), q, 2)
         srv.wait(warm, 60)
         srv.result(warm)
         srv.reset_stats()
-        log(f"served: {count} requests at {rate}/s (Poisson), arena {budget_gb} GiB")
 
         def submitter():
             t0 = time.perf_counter()
@@ -198,12 +196,39 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
         srv.drain(600)
         wall = time.perf_counter() - t0
         stats = srv.stats()
-        outs = {}
-        picks = list(range(0, count, max(1, count // sample)))[:sample]
+        outs, failed = {}, []
         for k, t in enumerate(tickets):
-            out, tt, nst, lat = srv.result(t)
-            if k in picks:
+            try:
+                out, tt, nst, lat = srv.result(t)
+            except SonicError as e:
+                # a request whose state turns non-physical fails alone; the
+                # reference's CPU path stops on the same problems
+                if e.code != -3:
+                    raise
+                failed.append(k)
+                continue
+            if k in keep:
                 outs[k] = out
+    return stats, wall, outs, failed
+
+
+def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int = 5005):
+    """Served stream of four-quadrant 2D Riemann problems (N log-uniform in
+    [64, 1024], 50-500 steps, WENO5/HLLC/SSPRK3) arriving as a Poisson
+    process; requests are admitted into the device arena by footprint as
+    memory frees up (continuous batching).  A first pass offers `rate`
+    requests/s (above what the GPU sustains) and measures solves/s; a second
+    pass offers 70% of that and measures the latency percentiles."""
+    from sonicsolver_amd import workloads as wl
+    reqs = [(kw, steps, make()) for _, kw, steps, make in wl.quadrant_requests(count, seed=seed)]
+    cells = sum(kw["n"][0] * kw["n"][1] * st for kw, st, _ in reqs)
+    picks = list(range(0, count, max(1, count // sample)))[:sample]
+    log(f"served: {count} requests at {rate}/s (Poisson), arena {budget_gb} GiB")
+    sat, sat_wall, outs, failed = _serve_pass(reqs, rate, budget_gb, seed + 1, set(picks))
+    moderate = round(0.7 * sat["solves_per_s"], 2)
+    log(f"served: {sat['solves_per_s']:.1f} solves/s at saturation; latency pass at {moderate}/s")
+    lat, lat_wall, _, _ = _serve_pass(reqs, moderate, budget_gb, seed + 2, set())
+    picks = [k for k in picks if k in outs]
     plain_s = 0.0
     worst = (0.0, 0.0)
     for k in picks:
@@ -216,22 +241,22 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
     return {
         "workload": f"{count} four-quadrant 2D Riemann requests, N log-uniform [64,1024], "
                     "50-500 steps, WENO5/HLLC/SSPRK3 FP64, Poisson arrivals",
-        "rate_per_s": rate,
-        "solves_per_s": round(stats["solves_per_s"], 2),
-        "p50_ms": round(stats["p50_ms"], 2),
-        "p99_ms": round(stats["p99_ms"], 2),
-        "mean_ms": round(stats["mean_ms"], 2),
-        "wall_s": round(wall, 3),
-        "mcell_updates_per_s": round(cells / wall / 1e6, 2),
-        "gpu_busy_frac": round(stats["busy_s"] / wall, 3),
-        "peak_resident": stats["peak_items"],
-        "completed": stats["completed"],
-        "failed": stats["failed"],
+        "solves_per_s": round(sat["solves_per_s"], 2),
+        "saturation_rate_per_s": rate,
+        "mcell_updates_per_s": round(cells / sat_wall / 1e6, 2),
+        "gpu_busy_frac": round(sat["busy_s"] / sat_wall, 3),
+        "peak_resident": sat["peak_items"],
+        "latency_rate_per_s": moderate,
+        "p50_ms": round(lat["p50_ms"], 2),
+        "p99_ms": round(lat["p99_ms"], 2),
+        "mean_ms": round(lat["mean_ms"], 2),
+        "completed": sat["completed"],
+        "failed_nonphysical": len(failed),
         "plain_single_case": {
             "solves_per_s": round(plain_solves, 2),
             "sample": f"{len(picks)} of the requests, each on its own grid, back to back",
         },
-        "speedup_vs_plain": round(stats["solves_per_s"] / plain_solves, 2),
+        "speedup_vs_plain": round(sat["solves_per_s"] / plain_solves, 2),
         "max_rel_diff_vs_plain": {"l1": worst[0], "linf": worst[1]},
     }
 
@@ -265,7 +290,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=300)
     ap.add_argument("--served", type=int, default=240,
                     help="requests in the served run (0 skips it)")
-    ap.add_argument("--served-rate", type=float, default=40.0, help="Poisson arrivals per s")
+    ap.add_argument("--served-rate", type=float, default=60.0,
+                    help="Poisson arrivals per s of the saturation pass")
     ap.add_argument("--served-budget-gb", type=float, default=24.0)
     ap.add_argument("--plain-sample", type=int, default=32,
                     help="problems solved one case at a time for the plain comparison")

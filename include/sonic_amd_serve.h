@@ -65,6 +65,10 @@ int sf_server_result(sf_server *server, long long ticket, double *soa, double *t
  * then SF_E_STATE). */
 int sf_server_drain(sf_server *server, double timeout_s);
 int sf_server_stats(sf_server *server, sf_serve_stats *stats);
+/* Launch shape of the batch (as sf_batch_set_tiles / _occupancy /
+ * _schedule); applied before the loop's next pass.  The default is
+ * xlen 255, ylen 32, 16 workgroups per CU, one launch per variant. */
+int sf_server_set_shape(sf_server *server, int xlen, int ylen, int blocks_per_cu, int fused);
 /* Forget completed latencies and counters (the queue is kept). */
 int sf_server_reset_stats(sf_server *server);
 

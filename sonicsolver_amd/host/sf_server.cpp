@@ -89,6 +89,9 @@ struct sf_server {
     bool have_first = false;
     Clock::time_point first_submit, last_done;
 
+    bool shape_pending = false;
+    int shape[4] = {255, 32, 16, 0};   // xlen, ylen, blocks per CU, fused
+
     // loop thread only
     std::deque<RequestPtr> waiting;
     std::vector<RequestPtr> running;
@@ -169,6 +172,12 @@ void loop(sf_server* s) {
             while (!s->incoming.empty()) {
                 s->waiting.push_back(std::move(s->incoming.front()));
                 s->incoming.pop_front();
+            }
+            if (s->shape_pending) {
+                sf_batch_set_tiles(s->batch, s->shape[0], s->shape[1]);
+                sf_batch_set_occupancy(s->batch, s->shape[2]);
+                sf_batch_set_schedule(s->batch, s->shape[3]);
+                s->shape_pending = false;
             }
         }
         admit(s);
@@ -263,6 +272,12 @@ int sf_server_create(int device, double gamma, size_t device_bytes, int max_item
         delete s;
         return rc;
     }
+    // the launch shape the autotuner picks for one-variant streams of
+    // mid-size problems: one launch per variant, long x tiles, a deep
+    // persistent grid
+    sf_batch_set_schedule(s->batch, 0);
+    sf_batch_set_tiles(s->batch, 255, 32);
+    sf_batch_set_occupancy(s->batch, 16);
     s->max_items = max_items;
     s->steps_per_tick = steps_per_tick;
     s->arena_nodes = nodes;
@@ -406,6 +421,20 @@ int sf_server_stats(sf_server* s, sf_serve_stats* st) {
         const double span = std::chrono::duration<double>(s->last_done - s->first_submit).count();
         st->solves_per_s = span > 0.0 ? (double)s->completed / span : 0.0;
     }
+    return 0;
+}
+
+int sf_server_set_shape(sf_server* s, int xlen, int ylen, int blocks_per_cu, int fused) {
+    if (!s) return fail(SF_E_ARG, "sf_server_set_shape: null server");
+    if ((xlen != 63 && xlen != 127 && xlen != 255) || ylen < 4 || ylen > 4096 ||
+        blocks_per_cu < 1 || blocks_per_cu > 64)
+        return fail(SF_E_ARG, "sf_server_set_shape: shape out of range (see sf_batch_set_tiles)");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->shape[0] = xlen;
+    s->shape[1] = ylen;
+    s->shape[2] = blocks_per_cu;
+    s->shape[3] = fused ? 1 : 0;
+    s->shape_pending = true;
     return 0;
 }
 

@@ -50,6 +50,7 @@ def _lib():
             "sf_server_drain": [_VP, ctypes.c_double],
             "sf_server_stats": [_VP, ctypes.POINTER(ServeStats)],
             "sf_server_reset_stats": [_VP],
+            "sf_server_set_shape": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)
@@ -125,3 +126,8 @@ class Server:
 
     def reset_stats(self):
         _check(self._lib.sf_server_reset_stats(self._h))
+
+    def set_shape(self, xlen: int = 255, ylen: int = 32, blocks_per_cu: int = 16,
+                  fused: bool = False):
+        _check(self._lib.sf_server_set_shape(self._h, int(xlen), int(ylen), int(blocks_per_cu),
+                                             1 if fused else 0))
