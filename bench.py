@@ -295,7 +295,22 @@ def main():
     ap.add_argument("--served-budget-gb", type=float, default=24.0)
     ap.add_argument("--plain-sample", type=int, default=32,
                     help="problems solved one case at a time for the plain comparison")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a device copy of known size (2 GiB read + 2 GiB written, "
+                         "5 times) to calibrate HBM traffic counters, then exit")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = 1 << 28
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
+                          "launches": 5}))
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
