@@ -194,9 +194,11 @@ __global__ __launch_bounds__(BS) void k_batch_x_fused(BatchDev B, const double* 
     }
 }
 
-// One launch per variant (fewer registers than the fused form).
+// One launch per variant (fewer registers than the fused form).  Three waves
+// per SIMD fit without spilling except for TENO5 + HLLC, which keeps two.
 template <int BS, int SCHEME, int FLUX>
-__global__ __launch_bounds__(BS) void k_batch_x(BatchDev B, const double* __restrict__ q,
+__global__ __launch_bounds__(BS)
+__attribute__((amdgpu_waves_per_eu((SCHEME == 1 && FLUX == 1) ? 2 : 3, 8))) void k_batch_x(BatchDev B, const double* __restrict__ q,
                                                 const double* __restrict__ S, BatchStage st,
                                                 int t0, int t1) {
     __shared__ double sh[5][BS];

@@ -34,8 +34,11 @@ namespace sfamd {
 // ---------------------------------------------------------------------------
 constexpr int kXBlock = 128;
 
+// The final WENO5 sweeps fit four waves per SIMD (128 VGPRs) without
+// spilling when asked to; the others stay at three.
 template <int SCHEME, int FLUX, int MODE>
-__global__ __launch_bounds__(kXBlock) void k_sweep_x(GridDev G, const double* __restrict__ q,
+__global__ __launch_bounds__(kXBlock)
+__attribute__((amdgpu_waves_per_eu((MODE == kFinal && SCHEME == 0) ? 4 : 3, 8))) void k_sweep_x(GridDev G, const double* __restrict__ q,
                                                      double* __restrict__ S, StageArgs A) {
     __shared__ double sh[5][kXBlock];
     __shared__ unsigned long long red[kXBlock / 64];
