@@ -125,6 +125,93 @@ __device__ __forceinline__ double teno5(double v0, double v1, double v2,
     return (0.1 * c0 * q0 + 0.6 * c1 * q1 + 0.3 * c2 * q2) / (6.0 * active);
 }
 
+// The left reconstruction at a face reads c0..c4 and the mirrored right one
+// c5..c1 (fillScalarWENOStencils<6>, SF_WENO.h:79-98).  Two of the three
+// candidate stencils are the same three nodes on both sides, so their
+// candidate values (the same parabola at the same face) and the
+// second-difference part of their smoothness indicators are formed once;
+// the two final divisions share one reciprocal.  Each side's weights are
+// those of weno5 / teno5 above.
+struct PairTerms {
+    double qL0, qR0, Sa, Sb;     // candidates (x6): left S0, right S0, shared
+    double bL[3], bR[3];         // smoothness indicators
+};
+
+__device__ __forceinline__ PairTerms pair_terms(double c0, double c1, double c2, double c3,
+                                                double c4, double c5) {
+    PairTerms t;
+    t.qL0 = 2.0 * c0 - 7.0 * c1 + 11.0 * c2;
+    t.qR0 = 2.0 * c5 - 7.0 * c4 + 11.0 * c3;
+    t.Sa = -c1 + 5.0 * c2 + 2.0 * c3;   // left S1 = right S2
+    t.Sb = 2.0 * c2 + 5.0 * c3 - c4;    // left S2 = right S1
+    const double dB = sq(c1 - 2.0 * c2 + c3);
+    const double dC = sq(c2 - 2.0 * c3 + c4);
+    t.bL[0] = 13.0 / 12.0 * sq(c0 - 2.0 * c1 + c2) + 0.25 * sq(c0 - 4.0 * c1 + 3.0 * c2);
+    t.bL[1] = 13.0 / 12.0 * dB + 0.25 * sq(c1 - c3);
+    t.bL[2] = 13.0 / 12.0 * dC + 0.25 * sq(3.0 * c2 - 4.0 * c3 + c4);
+    t.bR[0] = 13.0 / 12.0 * sq(c5 - 2.0 * c4 + c3) + 0.25 * sq(c5 - 4.0 * c4 + 3.0 * c3);
+    t.bR[1] = 13.0 / 12.0 * dC + 0.25 * sq(c4 - c2);
+    t.bR[2] = 13.0 / 12.0 * dB + 0.25 * sq(3.0 * c3 - 4.0 * c2 + c1);
+    return t;
+}
+
+// a/b and c/d with one reciprocal (both denominators are positive and far
+// from the ends of the exponent range here).
+__device__ __forceinline__ void div_pair(double a, double b, double c, double d, double& x,
+                                         double& y) {
+    const double r = 1.0 / (b * d);
+    x = a * d * r;
+    y = c * b * r;
+}
+
+__device__ __forceinline__ void weno5_pair(double c0, double c1, double c2, double c3,
+                                           double c4, double c5, double& left,
+                                           double& right) {
+    const PairTerms t = pair_terms(c0, c1, c2, c3, c4, c5);
+    const double eps = 1e-6;
+    const double eL0 = sq(eps + t.bL[0]), eL1 = sq(eps + t.bL[1]), eL2 = sq(eps + t.bL[2]);
+    const double eR0 = sq(eps + t.bR[0]), eR1 = sq(eps + t.bR[1]), eR2 = sq(eps + t.bR[2]);
+    const double wL0 = 0.1 * (eL1 * eL2), wL1 = 0.6 * (eL0 * eL2), wL2 = 0.3 * (eL0 * eL1);
+    const double wR0 = 0.1 * (eR1 * eR2), wR1 = 0.6 * (eR0 * eR2), wR2 = 0.3 * (eR0 * eR1);
+    const double nL = wL0 * t.qL0 + wL1 * t.Sa + wL2 * t.Sb;
+    const double nR = wR0 * t.qR0 + wR1 * t.Sb + wR2 * t.Sa;
+    div_pair(nL, 6.0 * (wL0 + wL1 + wL2), nR, 6.0 * (wR0 + wR1 + wR2), left, right);
+}
+
+__device__ __forceinline__ void teno5_weights(const double b[3], double& c0, double& c1,
+                                              double& c2) {
+    const double tau5 = fabs(b[0] - b[2]);
+    const double g0 = pow6(1.0 + tau5 / (b[0] + 1.0e-40));
+    const double g1 = pow6(1.0 + tau5 / (b[1] + 1.0e-40));
+    const double g2 = pow6(1.0 + tau5 / (b[2] + 1.0e-40));
+    const double gs = g0 + g1 + g2;
+    c0 = teno_cut(g0, gs) ? 0.0 : 1.0;
+    c1 = teno_cut(g1, gs) ? 0.0 : 1.0;
+    c2 = teno_cut(g2, gs) ? 0.0 : 1.0;
+}
+
+__device__ __forceinline__ void teno5_pair(double c0, double c1, double c2, double c3,
+                                           double c4, double c5, double& left, double& right,
+                                           unsigned& flags) {
+    const PairTerms t = pair_terms(c0, c1, c2, c3, c4, c5);
+    double l0, l1, l2, r0, r1, r2;
+    teno5_weights(t.bL, l0, l1, l2);
+    teno5_weights(t.bR, r0, r1, r2);
+    const double aL = 0.1 * l0 + 0.6 * l1 + 0.3 * l2;
+    const double aR = 0.1 * r0 + 0.6 * r1 + 0.3 * r2;
+    if (!(aL > 0.0) || !(aR > 0.0)) flags |= kFlagTeno;
+    const double nL = 0.1 * l0 * t.qL0 + 0.6 * l1 * t.Sa + 0.3 * l2 * t.Sb;
+    const double nR = 0.1 * r0 * t.qR0 + 0.6 * r1 * t.Sb + 0.3 * r2 * t.Sa;
+    div_pair(nL, 6.0 * aL, nR, 6.0 * aR, left, right);
+}
+
+template <int SCHEME>
+__device__ __forceinline__ void recon_pair(const double ch[6], double& left, double& right,
+                                           unsigned& flags) {
+    if constexpr (SCHEME == 1) teno5_pair(ch[0], ch[1], ch[2], ch[3], ch[4], ch[5], left, right, flags);
+    else weno5_pair(ch[0], ch[1], ch[2], ch[3], ch[4], ch[5], left, right);
+}
+
 template <int SCHEME>
 __device__ __forceinline__ double recon(double v0, double v1, double v2,
                                         double v3, double v4, unsigned& flags) {
@@ -258,87 +345,51 @@ __device__ __forceinline__ void face_flux(const double st[6][5], double gamma,
     const double un = u[nd];
     const double ul = sl * u[li];
     const double um = sm * u[mi];
-    const double b1 = (gamma - 1.0) / (c * c);
+    const double rc = 1.0 / c;
+    const double b1 = (gamma - 1.0) * (rc * rc);
     const double b2 = 0.5 * b1;
-    const double x1 = 1.0 / (2.0 * c);
+    const double x1 = 0.5 * rc;
 
-    // rows 0, 3, 4 of L (rows 1, 2 are sparse on an axis-aligned face)
-    const double L00 = 1.0 - b2 * q2;
-    const double L0[3] = {b1 * u[0], b1 * u[1], b1 * u[2]};
-    const double L04 = -b1;
-    const double A3 = 0.5 * b2 * q2 + x1 * un;
-    const double A4 = 0.5 * b2 * q2 - x1 * un;
-    double C3[3], C4[3];
+    // Characteristic projections (gemv5 with L, SF_riemann.h:33-37).  With
+    // P = b2 (q2/2 rho - u.m + E) and X = x1 (un rho - m_n), the rows of L
+    // give w0 = rho - 2P (acoustic-free row), w3 = P + X, w4 = P - X, and
+    // the shear rows w1 = sl (m_l - u_l rho), w2 = sm (m_m - u_m rho).
+    const double hq2 = 0.5 * q2;
+    double ch[5][6];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        C3[d] = (d == nd) ? -x1 - b2 * u[d] : -(b2 * u[d]);
-        C4[d] = (d == nd) ? x1 - b2 * u[d] : -(b2 * u[d]);
+    for (int s = 0; s < 6; ++s) {
+        const double r = st[s][0];
+        const double um_dot = u[0] * st[s][1] + u[1] * st[s][2] + u[2] * st[s][3];
+        const double P = b2 * (hq2 * r - um_dot + st[s][4]);
+        const double X = x1 * (un * r - st[s][1 + nd]);
+        ch[0][s] = r - 2.0 * P;
+        ch[1][s] = -ul * r + sl * st[s][1 + li];
+        ch[2][s] = -um * r + sm * st[s][1 + mi];
+        ch[3][s] = P + X;
+        ch[4][s] = P - X;
     }
-
-    // Characteristic projections (gemv5 with L, SF_riemann.h:33-37) and
-    // the scalar reconstructions: left stencil s = 0..4, right stencil
-    // mirrored s = 5..1 (fillScalarWENOStencils<6>, SF_WENO.h:79-98).
     double wL[5], wR[5];
-    {
-        double ch[6];
 #pragma unroll
-        for (int s = 0; s < 6; ++s)
-            ch[s] = L00 * st[s][0] + L0[0] * st[s][1] + L0[1] * st[s][2]
-                  + L0[2] * st[s][3] + L04 * st[s][4];
-        wL[0] = recon<SCHEME>(ch[0], ch[1], ch[2], ch[3], ch[4], flags);
-        wR[0] = recon<SCHEME>(ch[5], ch[4], ch[3], ch[2], ch[1], flags);
-    }
-    {
-        double ch[6];
-#pragma unroll
-        for (int s = 0; s < 6; ++s) ch[s] = -ul * st[s][0] + sl * st[s][1 + li];
-        wL[1] = recon<SCHEME>(ch[0], ch[1], ch[2], ch[3], ch[4], flags);
-        wR[1] = recon<SCHEME>(ch[5], ch[4], ch[3], ch[2], ch[1], flags);
-    }
-    {
-        double ch[6];
-#pragma unroll
-        for (int s = 0; s < 6; ++s) ch[s] = -um * st[s][0] + sm * st[s][1 + mi];
-        wL[2] = recon<SCHEME>(ch[0], ch[1], ch[2], ch[3], ch[4], flags);
-        wR[2] = recon<SCHEME>(ch[5], ch[4], ch[3], ch[2], ch[1], flags);
-    }
-    {
-        double ch[6];
-#pragma unroll
-        for (int s = 0; s < 6; ++s)
-            ch[s] = A3 * st[s][0] + C3[0] * st[s][1] + C3[1] * st[s][2]
-                  + C3[2] * st[s][3] + b2 * st[s][4];
-        wL[3] = recon<SCHEME>(ch[0], ch[1], ch[2], ch[3], ch[4], flags);
-        wR[3] = recon<SCHEME>(ch[5], ch[4], ch[3], ch[2], ch[1], flags);
-    }
-    {
-        double ch[6];
-#pragma unroll
-        for (int s = 0; s < 6; ++s)
-            ch[s] = A4 * st[s][0] + C4[0] * st[s][1] + C4[1] * st[s][2]
-                  + C4[2] * st[s][3] + b2 * st[s][4];
-        wL[4] = recon<SCHEME>(ch[0], ch[1], ch[2], ch[3], ch[4], flags);
-        wR[4] = recon<SCHEME>(ch[5], ch[4], ch[3], ch[2], ch[1], flags);
-    }
+    for (int v = 0; v < 5; ++v) recon_pair<SCHEME>(ch[v], wL[v], wR[v], flags);
 
-    // Back to conservative variables with R (gemv5).
+    // Back to conservative variables with R (gemv5).  Every column of R
+    // carries u in the momentum rows except the acoustic +-c on the normal
+    // and the shear units, so q_m = u q_rho + (the rest).
     double qL[5], qR[5];
-    const double Hm = H - c * un, Hp = H + c * un;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const double* w = side ? wR : wL;
         double* q = side ? qR : qL;
-        q[0] = w[0] + w[3] + w[4];
+        const double r = w[0] + w[3] + w[4];
+        const double dw = w[4] - w[3];
+        q[0] = r;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            if (d == nd)
-                q[1 + d] = u[d] * w[0] + (u[d] - c) * w[3] + (u[d] + c) * w[4];
-            else if (d == li)
-                q[1 + d] = u[d] * w[0] + sl * w[1] + u[d] * w[3] + u[d] * w[4];
-            else
-                q[1 + d] = u[d] * w[0] + sm * w[2] + u[d] * w[3] + u[d] * w[4];
+            if (d == nd) q[1 + d] = u[d] * r + c * dw;
+            else if (d == li) q[1 + d] = u[d] * r + sl * w[1];
+            else q[1 + d] = u[d] * r + sm * w[2];
         }
-        q[4] = 0.5 * q2 * w[0] + ul * w[1] + um * w[2] + Hm * w[3] + Hp * w[4];
+        q[4] = hq2 * w[0] + ul * w[1] + um * w[2] + H * (w[3] + w[4]) + (c * un) * dw;
     }
 
     double f[5];
