@@ -44,6 +44,36 @@ template <> struct Basis<2> {
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
+// Reciprocal, quotient and reciprocal square root from the hardware
+// estimate refined by two Newton steps: within about one ulp of the
+// correctly rounded value for the normal, finite, nonzero operands of this
+// path (a zero or non-finite operand yields NaN, which the flux and state
+// checks report).  The IEEE sequences hipcc emits for '/' and sqrt are two
+// to three times longer.
+__device__ __forceinline__ double frcp(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    r = fma(fma(-b, r, 1.0), r, r);
+    return r;
+}
+
+__device__ __forceinline__ double fdiv(double a, double b) {
+    const double r = frcp(b);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+
+__device__ __forceinline__ double frsqrt(double z) {
+    double y = __builtin_amdgcn_rsq(z);
+    const double h = 0.5 * z;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
+// sqrt(z) as z * rsqrt(z) (z > 0)
+__device__ __forceinline__ double fsqrt(double z) { return z * frsqrt(z); }
+
 // WENO5::weno5_core, src/methods/numerics/convection/SF_WENO5.cpp:42-62.
 // Same weights and candidates, evaluated with one division: with
 // e_k = (eps + beta_k)^2 and alpha_k = d_k / e_k,
@@ -159,7 +189,7 @@ __device__ __forceinline__ PairTerms pair_terms(double c0, double c1, double c2,
 // from the ends of the exponent range here).
 __device__ __forceinline__ void div_pair(double a, double b, double c, double d, double& x,
                                          double& y) {
-    const double r = 1.0 / (b * d);
+    const double r = frcp(b * d);
     x = a * d * r;
     y = c * b * r;
 }
@@ -181,9 +211,9 @@ __device__ __forceinline__ void weno5_pair(double c0, double c1, double c2, doub
 __device__ __forceinline__ void teno5_weights(const double b[3], double& c0, double& c1,
                                               double& c2) {
     const double tau5 = fabs(b[0] - b[2]);
-    const double g0 = pow6(1.0 + tau5 / (b[0] + 1.0e-40));
-    const double g1 = pow6(1.0 + tau5 / (b[1] + 1.0e-40));
-    const double g2 = pow6(1.0 + tau5 / (b[2] + 1.0e-40));
+    const double g0 = pow6(1.0 + fdiv(tau5, b[0] + 1.0e-40));
+    const double g1 = pow6(1.0 + fdiv(tau5, b[1] + 1.0e-40));
+    const double g2 = pow6(1.0 + fdiv(tau5, b[2] + 1.0e-40));
     const double gs = g0 + g1 + g2;
     c0 = teno_cut(g0, gs) ? 0.0 : 1.0;
     c1 = teno_cut(g1, gs) ? 0.0 : 1.0;
@@ -229,14 +259,14 @@ struct Closed {
 __device__ __forceinline__ Closed close_state(const double q[5], double gamma) {
     Closed s;
     const double rho = q[0];
-    s.inv_rho = 1.0 / rho;
+    s.inv_rho = frcp(rho);
     s.u[0] = q[1] * s.inv_rho;
     s.u[1] = q[2] * s.inv_rho;
     s.u[2] = q[3] * s.inv_rho;
     const double kinetic = 0.5 * rho * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
     const double e = (q[4] - kinetic) * s.inv_rho;
     s.p = (gamma - 1.0) * rho * e;
-    s.c = sqrt(gamma * s.p * s.inv_rho);
+    s.c = fsqrt(gamma * s.p * s.inv_rho);
     return s;
 }
 
@@ -274,18 +304,18 @@ __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
                                      double gamma, double out[5]) {
     const Closed l = close_state(qL, gamma), r = close_state(qR, gamma);
     const double unL = l.u[ND], unR = r.u[ND];
-    const double sL = sqrt(qL[0]), sR = sqrt(qR[0]);
-    const double inv = 1.0 / (sL + sR);
+    const double sL = fsqrt(qL[0]), sR = fsqrt(qR[0]);
+    const double inv = frcp(sL + sR);
     double ut[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) ut[d] = (sL * l.u[d] + sR * r.u[d]) * inv;
     const double HL = (qL[4] + l.p) * l.inv_rho, HR = (qR[4] + r.p) * r.inv_rho;
     const double Ht = (sL * HL + sR * HR) * inv;
-    const double ct = sqrt((gamma - 1.0) * (Ht - 0.5 * (ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2])));
+    const double ct = fsqrt((gamma - 1.0) * (Ht - 0.5 * (ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2])));
     const double SL = fmin(unL - l.c, ut[ND] - ct);
     const double SR = fmax(unR + r.c, ut[ND] + ct);
     const double mL = qL[0] * (SL - unL), mR = qR[0] * (SR - unR);
-    const double Ss = (r.p - l.p + unL * mL - unR * mR) / (mL - mR);
+    const double Ss = fdiv(r.p - l.p + unL * mL - unR * mR, mL - mR);
     if (0.0 <= SL) {
         physical_flux<ND>(qL, l, out);
         return;
@@ -304,13 +334,15 @@ __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
     const double mK = left ? mL : mR;
     double f[5];
     physical_flux<ND>(q, s, f);
-    const double scale = mK / (SK - Ss);
+    // scale = mK / (SK - Ss); the pressure term p / mK * scale = p / (SK - Ss)
+    const double rs = frcp(SK - Ss);
+    const double scale = mK * rs;
     double us[5];
     us[0] = scale;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
         us[1 + d] = scale * ((d == ND) ? s.u[d] + (Ss - un) : s.u[d]);
-    us[4] = scale * (q[4] * s.inv_rho + (Ss - un) * (Ss + s.p / mK));
+    us[4] = scale * (q[4] * s.inv_rho + (Ss - un) * Ss) + (Ss - un) * (s.p * rs);
 #pragma unroll
     for (int v = 0; v < 5; ++v) out[v] = f[v] + SK * (us[v] - q[v]);
 }
@@ -331,7 +363,7 @@ __device__ __forceinline__ void face_flux(const double st[6][5], double gamma,
     double qa[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) qa[v] = 0.5 * (st[2][v] + st[3][v]);
-    const double invR = 1.0 / qa[0];
+    const double invR = frcp(qa[0]);
     const double ke = 0.5 * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * invR;
     const double p = (gamma - 1.0) * (qa[4] - ke);
     if (!(qa[0] > 0.0) || !(p > 0.0)) flags |= kFlagChar;
@@ -340,12 +372,13 @@ __device__ __forceinline__ void face_flux(const double st[6][5], double gamma,
     u[1] = qa[2] * invR;
     u[2] = qa[3] * invR;
     const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const double c = sqrt(gamma * p * invR);
+    const double cz = gamma * p * invR;
+    const double rc = frsqrt(cz);
+    const double c = cz * rc;
     const double H = (qa[4] + p) * invR;
     const double un = u[nd];
     const double ul = sl * u[li];
     const double um = sm * u[mi];
-    const double rc = 1.0 / c;
     const double b1 = (gamma - 1.0) * (rc * rc);
     const double b2 = 0.5 * b1;
     const double x1 = 0.5 * rc;
