@@ -294,6 +294,9 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
         "saturation_rate_per_s": rate,
         "mcell_updates_per_s": round(cells / sat_wall / 1e6, 2),
         "gpu_busy_frac": round(sat["busy_s"] / sat_wall, 3),
+        "admit_frac": round(sat["admit_s"] / sat_wall, 3),
+        "retire_frac": round(sat["retire_s"] / sat_wall, 3),
+        "ticks": sat["ticks"],
         "peak_resident": sat["peak_items"],
         "latency_rate_per_s": moderate,
         "p50_ms": round(lat["p50_ms"], 2),

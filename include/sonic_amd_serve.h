@@ -34,6 +34,8 @@ typedef struct {
     double solves_per_s;     /* completed / (last completion - first submit)   */
     double cell_updates;     /* sum of nodes x steps over completed requests   */
     double busy_s;           /* time the loop spent advancing the batch        */
+    double admit_s;          /* time spent admitting (state upload included)   */
+    double retire_s;         /* time spent handing results back (download)     */
     long long ticks;         /* batch advances                                 */
     int peak_items;          /* most problems resident at once                 */
     size_t peak_nodes;       /* most arena nodes in use at once                */

@@ -42,9 +42,9 @@ struct BatchDev {
     const BatchItemDev* items;
     long long P;        // arena capacity in padded nodes (per variable)
     double gamma;
-    const int4* xtiles;  // (slot, line j, first node i0, -)
+    const int4* xtiles;  // (slot, first face f0, -, -)
     int nxt;
-    const int4* ytiles;  // (slot, first column i0, first node a0, -)
+    const int4* ytiles;  // (slot, first (segment, column) index, -, -)
     int nyt;
     int yseg;            // nodes per y tile
     const int* active;   // admitted slots
@@ -102,10 +102,13 @@ __device__ __forceinline__ bool item_solved(const BatchItemDev& it, int i, int j
 }
 
 // x faces + SSPRK3 stage update over tiles [t0, t1) of one (scheme, flux)
-// variant.  A tile is BS-1 nodes of one x line: lane t evaluates face
-// (i0-1+t)+1/2, lanes 1..BS-1 difference it with their left neighbour's face
-// through LDS and update node i0-1+t.  Block b0 of nb takes tiles
-// t0+b0, t0+b0+nb, ... (persistent).
+// variant.  A problem's x faces are numbered line by line, f = j*(nx+1) +
+// (i+1) for face i+1/2, i = -1..nx-1, and a tile is BS consecutive faces
+// starting at f0 (tiles overlap by one face): lane t evaluates face f0+t,
+// and lanes 1..BS-1 whose face is not the first of its line difference it
+// with their left neighbour's face through LDS and update that node.  Short
+// lines therefore share a block and long ones span several.  Block b0 of nb
+// takes tiles t0+b0, t0+b0+nb, ... (persistent).
 template <int BS, int SCHEME, int FLUX>
 __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restrict__ q,
                                         const double* __restrict__ S, const BatchStage& st,
@@ -117,26 +120,29 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
         const int4 T = B.xtiles[tile];
         if (!item_live(B, T.x)) continue;  // uniform over the block
         const BatchItemDev it = B.items[T.x];
-        const int j = T.y;
-        const int a = T.z - 1 + t;
-        if (a <= it.n0 - 1) {
+        const int per_line = it.n0 + 1;
+        const int f = T.y + t;
+        const bool face = f < per_line * it.n1;
+        const int j = face ? f / per_line : 0;
+        const int a = f - j * per_line - 1;  // lower node of the face
+        if (face) {
             const long long base = item_index(it, a, j);
             double stc[6][5];
 #pragma unroll
             for (int s = 0; s < 6; ++s)
 #pragma unroll
                 for (int v = 0; v < 5; ++v) stc[s][v] = q[v * B.P + base + (s - 2)];
-            double f[5];
-            face_flux<0, SCHEME, FLUX>(stc, B.gamma, it.area0, f, flags);
+            double f5[5];
+            face_flux<0, SCHEME, FLUX>(stc, B.gamma, it.area0, f5, flags);
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
-                if (!isfinite(f[v])) flags |= kFlagFlux;
-                sh[v][t] = f[v];
+                if (!isfinite(f5[v])) flags |= kFlagFlux;
+                sh[v][t] = f5[v];
             }
         }
         __syncthreads();
         double dl = INFINITY;
-        if (t >= 1 && a <= it.n0 - 1) {
+        if (t >= 1 && face && a >= 0) {
             const long long id = item_index(it, a, j);
             double s5[5];
 #pragma unroll
@@ -206,10 +212,12 @@ __attribute__((amdgpu_waves_per_eu((SCHEME == 1 && FLUX == 1) ? 2 : 3, 8))) void
     x_tiles<BS, SCHEME, FLUX>(B, q, S, st, t0, t1, blockIdx.x, gridDim.x, sh, red);
 }
 
-// y faces of the 2D problems over tiles [t0, t1) of one variant: a tile is
-// 64 columns by yseg nodes; each lane marches up its column with the 6-node
-// stencil window in LDS (one load per node) and writes
-// S = F(j+1/2) - F(j-1/2).
+// y faces of the 2D problems over tiles [t0, t1) of one variant.  A
+// problem's y work is numbered (segment, column) = k -> (k / nx, k % nx),
+// segments of yseg nodes; a tile is 64 consecutive k, one per lane, so
+// narrow problems fill whole waves.  Each lane marches up its column
+// segment with the 6-node stencil window in LDS (one load per node) and
+// writes S = F(j+1/2) - F(j-1/2).
 template <int SCHEME, int FLUX>
 __device__ __forceinline__ void y_tiles(const BatchDev& B, const double* __restrict__ q,
                                         double* __restrict__ S, int t0, int t1, int b0, int nb,
@@ -220,10 +228,13 @@ __device__ __forceinline__ void y_tiles(const BatchDev& B, const double* __restr
         const int4 T = B.ytiles[tile];
         if (!item_live(B, T.x)) continue;
         const BatchItemDev it = B.items[T.x];
-        const int i = T.y + lane;
-        const int a0 = T.z;
+        const int k = T.y + lane;
+        const int nseg = (it.n1 + B.yseg - 1) / B.yseg;
+        if (k >= nseg * it.n0) continue;  // per lane; no barrier below
+        const int seg = k / it.n0;
+        const int i = k - seg * it.n0;
+        const int a0 = seg * B.yseg;
         const int a1 = min(a0 + B.yseg, it.n1);
-        if (i >= it.n0) continue;  // per lane; no barrier below
         const long long step = it.m0;
         // nodes a0-3 .. a0+1 into slots 1..5; the march starts at a0-1 so
         // every face is evaluated by the same instructions (see
@@ -519,7 +530,7 @@ int rebuild_tables(sf_batch* b) {
         return (long long)b->items[x].n0 * b->items[x].n1 >
                (long long)b->items[y].n0 * b->items[y].n1;
     });
-    const int xlen = b->xbs - 1;
+    const int xstep = b->xbs - 1;
     int v = 0;
     b->xr[0] = b->yr[0] = 0;
     for (int s : order) {
@@ -529,12 +540,13 @@ int rebuild_tables(sf_batch* b) {
             b->yr[v] = (int)yt.size();
         }
         const BatchItemDev& it = b->items[s];
-        for (int j = 0; j < it.n1; ++j)
-            for (int i0 = 0; i0 < it.n0; i0 += xlen) xt.push_back(make_int4(s, j, i0, 0));
+        // x: tiles of xbs consecutive faces, overlapping by one
+        const long long faces = (long long)(it.n0 + 1) * it.n1;
+        for (long long f0 = 0; f0 < faces - 1; f0 += xstep) xt.push_back(make_int4(s, (int)f0, 0, 0));
         if (it.n1 > 1) {
             b->any2d = true;
-            for (int a0 = 0; a0 < it.n1; a0 += b->yseg)
-                for (int i0 = 0; i0 < it.n0; i0 += 64) yt.push_back(make_int4(s, i0, a0, 0));
+            const long long work = (long long)((it.n1 + b->yseg - 1) / b->yseg) * it.n0;
+            for (long long k0 = 0; k0 < work; k0 += 64) yt.push_back(make_int4(s, (int)k0, 0, 0));
         }
     }
     while (v < 4) {

@@ -82,7 +82,7 @@ struct sf_server {
     bool stop = false;
     long long submitted = 0, completed = 0, failed = 0, outstanding = 0, resident = 0;
     std::vector<double> latencies_ms;
-    double cell_updates = 0.0, busy_s = 0.0;
+    double cell_updates = 0.0, busy_s = 0.0, admit_s = 0.0, retire_s = 0.0;
     long long ticks = 0;
     int peak_items = 0;
     size_t peak_nodes = 0;
@@ -180,7 +180,13 @@ void loop(sf_server* s) {
                 s->shape_pending = false;
             }
         }
+        const auto ta = Clock::now();
         admit(s);
+        {
+            const double d = std::chrono::duration<double>(Clock::now() - ta).count();
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->admit_s += d;
+        }
         if (s->running.empty()) {
             // nothing fits and nothing runs: only possible after failures;
             // wait for new work rather than spin
@@ -230,6 +236,7 @@ void loop(sf_server* s) {
             drop_running(s, "batch clock failed: " + last_error());
             continue;
         }
+        const auto tr = Clock::now();
         for (auto it = s->running.begin(); it != s->running.end();) {
             const RequestPtr r = *it;
             r->steps_done = steps[(size_t)r->slot];
@@ -246,6 +253,9 @@ void loop(sf_server* s) {
             it = s->running.erase(it);
             finish(s, r, rc2 == 0 ? 1 : -1, err);
         }
+        const double d = std::chrono::duration<double>(Clock::now() - tr).count();
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->retire_s += d;
     }
 }
 
@@ -400,6 +410,8 @@ int sf_server_stats(sf_server* s, sf_serve_stats* st) {
     st->queued = s->outstanding - s->resident;
     st->cell_updates = s->cell_updates;
     st->busy_s = s->busy_s;
+    st->admit_s = s->admit_s;
+    st->retire_s = s->retire_s;
     st->ticks = s->ticks;
     st->peak_items = s->peak_items;
     st->peak_nodes = s->peak_nodes;
@@ -444,7 +456,7 @@ int sf_server_reset_stats(sf_server* s) {
     s->submitted = s->outstanding;
     s->completed = s->failed = 0;
     s->latencies_ms.clear();
-    s->cell_updates = s->busy_s = 0.0;
+    s->cell_updates = s->busy_s = s->admit_s = s->retire_s = 0.0;
     s->ticks = 0;
     s->peak_items = 0;
     s->peak_nodes = 0;

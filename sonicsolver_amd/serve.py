@@ -24,7 +24,8 @@ class ServeStats(ctypes.Structure):
                 ("running", _LL), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
                 ("solves_per_s", ctypes.c_double), ("cell_updates", ctypes.c_double),
-                ("busy_s", ctypes.c_double), ("ticks", _LL), ("peak_items", ctypes.c_int),
+                ("busy_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("retire_s", ctypes.c_double), ("ticks", _LL), ("peak_items", ctypes.c_int),
                 ("peak_nodes", ctypes.c_size_t), ("arena_nodes", ctypes.c_size_t)]
 
     def as_dict(self):
