@@ -113,6 +113,9 @@ int check_desc(const sf_grid_desc* d) {
         return fail(SF_E_ARG, "unknown time recipe");
     const long long N = (long long)d->n[0] * d->n[1] * d->n[2];
     if (N > (1LL << 31)) return fail(SF_E_ARG, "grid exceeds 2^31 nodes");
+    // the x sweep numbers faces in 32 bits
+    if ((long long)(d->n[0] + 1) * d->n[1] * d->n[2] >= (1LL << 32) - 256)
+        return fail(SF_E_ARG, "grid has 2^32 - 2^8 or more x faces");
     return 0;
 }
 
@@ -178,12 +181,11 @@ void launch_axis(sf_grid* g, int axis, const double* q, const StageArgs& A) {
     const GridDev& G = g->G;
     const int ev = prof_start(g, SF_KERNEL_SWEEP_X + axis);
     if (axis == 0) {
-        // a few thousand blocks in total, each looping over x lines
-        const unsigned bx = blocks_for(G.n[0], kXBlock - 1);
-        const unsigned lines = (unsigned)G.n[1] * (unsigned)G.n[2];
-        const unsigned by = std::max(1u, std::min(lines, 16384u / bx));
-        dim3 grid(bx, by, 1);
-        k_sweep_x<SCHEME, FLUX, MODE><<<grid, kXBlock, 0, g->stream>>>(G, q, g->S, A);
+        // up to 16384 blocks, each looping over tiles of 127 faces
+        const long long faces = (long long)(G.n[0] + 1) * G.n[1] * G.n[2];
+        const long long tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
+        const unsigned bx = (unsigned)std::min<long long>(tiles, 16384);  // faces < 2^32 - 2^8
+        k_sweep_x<SCHEME, FLUX, MODE><<<bx, kXBlock, 0, g->stream>>>(G, q, g->S, A);
         prof_stop(g, ev);
         return;
     }

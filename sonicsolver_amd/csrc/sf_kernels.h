@@ -27,46 +27,57 @@
 namespace sfamd {
 
 // ---------------------------------------------------------------------------
-// x sweep: lane t of a 128-lane block evaluates face (i0-1+t)+1/2; lanes
-// 1..127 then form the difference for node i0-1+t.  Blocks cover 127 nodes
-// of an x line and loop over lines (blockIdx.y, stride gridDim.y), so the
-// CFL fold costs one atomic per block rather than per line.
+// x sweep over the faces of all x lines numbered as one sequence: face
+// f = line * (n0 + 1) + (a + 1) lies between nodes a and a+1 of its line
+// (a = -1 .. n0-1).  Tile T is faces T*127 .. T*127+127: lane t evaluates
+// face T*127+t and lanes 1..127 form the difference for the node whose upper
+// face is theirs.  Tiles overlap by one face, so every node is updated by
+// exactly one lane, and no tile is left mostly idle at the end of a line.
+// Blocks loop over tiles (stride gridDim.x), so the CFL fold costs one
+// atomic per block.
 // ---------------------------------------------------------------------------
 constexpr int kXBlock = 128;
 
-// The final WENO5 sweeps fit four waves per SIMD (128 VGPRs) without
-// spilling when asked to; the others stay at three.
+// The final HLLC sweeps fit four waves per SIMD (128 VGPRs) without
+// spilling when asked to; the others stay at three (the final Rusanov sweeps
+// would spill at four).
 template <int SCHEME, int FLUX, int MODE>
 __global__ __launch_bounds__(kXBlock)
-__attribute__((amdgpu_waves_per_eu((MODE == kFinal && SCHEME == 0) ? 4 : 3, 8))) void k_sweep_x(GridDev G, const double* __restrict__ q,
+__attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1) ? 4 : 3, 8))) void k_sweep_x(GridDev G, const double* __restrict__ q,
                                                      double* __restrict__ S, StageArgs A) {
     __shared__ double sh[5][kXBlock];
     __shared__ unsigned long long red[kXBlock / 64];
+    // face numbers fit 32 bits: the host rejects grids with 2^32 - 2^8
+    // faces or more along x
     const int t = threadIdx.x;
-    const int i0 = blockIdx.x * (kXBlock - 1);
-    const int a = i0 - 1 + t;  // lower node of this lane's face
-    const int lines = G.n[1] * G.n[2];
+    const unsigned per_line = G.n[0] + 1;
+    const unsigned faces = per_line * G.n[1] * G.n[2];
+    const unsigned tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
     unsigned flags = 0;
     double dt_local = INFINITY;
-    for (int line = blockIdx.y; line < lines; line += gridDim.y) {
-        const int j = line % G.n[1], k = line / G.n[1];
-        if (a <= G.n[0] - 1) {
+    for (unsigned T = blockIdx.x; T < tiles; T += gridDim.x) {
+        const unsigned f = T * (kXBlock - 1) + t;
+        const bool face = f < faces;
+        const unsigned line = face ? f / per_line : 0;
+        const int a = face ? (int)(f - line * per_line) - 1 : 0;  // lower node of the face
+        const int j = (int)(line % (unsigned)G.n[1]), k = (int)(line / (unsigned)G.n[1]);
+        if (face) {
             const long long base = node_index(G, a, j, k);
             double st[6][5];
 #pragma unroll
             for (int s = 0; s < 6; ++s)
 #pragma unroll
                 for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + base + (s - 2)];
-            double f[5];
-            face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f, flags);
+            double f5[5];
+            face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags);
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
-                if (!isfinite(f[v])) flags |= kFlagFlux;
-                sh[v][t] = f[v];
+                if (!isfinite(f5[v])) flags |= kFlagFlux;
+                sh[v][t] = f5[v];
             }
         }
         __syncthreads();
-        if (t >= 1 && a <= G.n[0] - 1) {
+        if (t >= 1 && face && a >= 0) {
             const long long id = node_index(G, a, j, k);
             double s5[5];
 #pragma unroll
@@ -84,7 +95,7 @@ __attribute__((amdgpu_waves_per_eu((MODE == kFinal && SCHEME == 0) ? 4 : 3, 8)))
                 for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
             }
         }
-        __syncthreads();  // sh is rewritten by the next line
+        __syncthreads();  // sh is rewritten by the next tile
     }
     raise_flags(flags, A.flags);
     if (MODE == kFinal && A.reduce_dt) block_min_atomic(dt_local, A.dtmin, red);
