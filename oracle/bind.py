@@ -23,10 +23,12 @@ REF_FMA_SO = HERE / "_ref" / "libsfref_fma.so"
 _P = ctypes.POINTER(ctypes.c_double)
 _U = ctypes.POINTER(ctypes.c_uint)
 
-SCHEMES = {"weno5": 0, "teno5": 1}
-FLUXES = {"rusanov": 0, "hllc": 1}
+SCHEMES = {"weno5": 0, "teno5": 1, "weno3": 2, "weno7": 3}
+FLUXES = {"rusanov": 0, "hllc": 1, "steger": 2, "laxfriedrichs": 3, "roe": 4}
 TIMES = {"euler": 0, "forwardeuler": 0, "ssprk3": 1, "rk4": 2, "classicalrk4": 2}
-BCS = {"transmissive": 0, "periodic": 1}
+# per axis; "none_lo" / "none_hi" / "none": that end carries no condition
+BCS = {"transmissive": 0, "periodic": 1, "none_lo": 2, "none_hi": 3, "none": 4}
+STENCIL = {0: 6, 1: 6, 2: 4, 3: 8}
 
 
 class Desc(ctypes.Structure):
@@ -85,6 +87,16 @@ def oracle():
         lib.sfo_weno5_core.restype = ctypes.c_double
         lib.sfo_teno5_core.argtypes = [_P, _U]
         lib.sfo_teno5_core.restype = ctypes.c_double
+        for name in ("sfo_weno3_core", "sfo_weno7_core"):
+            getattr(lib, name).argtypes = [_P, _U]
+            getattr(lib, name).restype = ctypes.c_double
+        lib.sfo_weno7_smoothness.argtypes = [_P, ctypes.c_double, ctypes.c_double]
+        lib.sfo_weno7_smoothness.restype = ctypes.c_double
+        for name in ("sfo_steger_warming", "sfo_lax_friedrichs", "sfo_roe"):
+            getattr(lib, name).argtypes = [_P, _P, _P, ctypes.c_double, _P]
+        lib.sfo_face_flux.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.c_double, _P]
+        lib.sfo_char_face_flux_n.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, _P]
         lib.sfo_char_matrix.argtypes = [_P, _P, ctypes.c_double, _P, _P]
         for name in ("sfo_rusanov", "sfo_hllc"):
             getattr(lib, name).argtypes = [_P, _P, _P, ctypes.c_double, _P]
@@ -136,6 +148,16 @@ def _declare_ref(lib):
     lib.sfref_rusanov.argtypes = [_P, _P, _P, ctypes.c_double, _P]
     lib.sfref_char_face_flux.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double, _P]
+    lib.sfref_char_face_flux_n.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_double, _P]
+    for name in ("sfref_weno3", "sfref_weno7"):
+        getattr(lib, name).argtypes = [_P]
+        getattr(lib, name).restype = ctypes.c_double
+    lib.sfref_split_flux.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.c_double, _P]
+    lib.sfref_run_driver.argtypes = [ctypes.POINTER(Desc), _P, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, _P,
+                                     ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     return lib
 
 
@@ -233,6 +255,23 @@ def ref_run(desc: Desc, q: np.ndarray, nsteps: int, cfl: float, max_dt: float = 
     return a, dts[:nsteps]
 
 
+def ref_run_driver(desc: Desc, q: np.ndarray, start: float, end: float, cfl: float,
+                   max_dt: float = 1e300, end_step: int = 0, write_time: float = 1e300,
+                   write_steps: int = 0, write_by_step: bool = False, cap: int = 100000,
+                   fma: bool = False):
+    """Time::Driver over the reference's own step (harness); returns (state,
+    dt sequence)."""
+    a = _arr(q).reshape(5, -1).copy()
+    dts = np.zeros(cap)
+    n = ctypes.c_int(0)
+    rc = ref(fma).sfref_run_driver(ctypes.byref(desc), _ptr(a), start, end, end_step,
+                                   int(write_by_step), write_steps, write_time, cfl, max_dt,
+                                   _ptr(dts), cap, ctypes.byref(n))
+    if rc != 0:
+        raise RuntimeError("the reference harness raised an error")
+    return a, dts[:n.value]
+
+
 def ref_residual(desc: Desc, q: np.ndarray):
     a = _arr(q).reshape(5, -1).copy()
     out = np.zeros_like(a)
@@ -252,6 +291,30 @@ def teno5(v):
     e = ctypes.c_uint(0)
     r = oracle().sfo_teno5_core(_ptr(_arr(v)), ctypes.byref(e))
     return r, e.value
+
+
+def weno3(v):
+    e = ctypes.c_uint(0)
+    r = oracle().sfo_weno3_core(_ptr(_arr(v)), ctypes.byref(e))
+    return r, e.value
+
+
+def weno7(v):
+    e = ctypes.c_uint(0)
+    r = oracle().sfo_weno7_core(_ptr(_arr(v)), ctypes.byref(e))
+    return r, e.value
+
+
+def weno7_smoothness(v, a, b):
+    return oracle().sfo_weno7_smoothness(_ptr(_arr(v)), float(a), float(b))
+
+
+def face_flux(flux, qL, qR, n, gamma=1.4):
+    """Any of the five face fluxes by name; (rc, flux)."""
+    out = np.zeros(5)
+    rc = oracle().sfo_face_flux(FLUXES[flux], _ptr(_arr(qL)), _ptr(_arr(qR)), _ptr(_arr(n)),
+                                gamma, _ptr(out))
+    return rc, out
 
 
 def char_matrix(q, n, gamma=1.4):
@@ -280,7 +343,10 @@ def physical_flux(q, n, gamma=1.4):
 
 
 def char_face_flux(stencil, cofactor, scheme="weno5", flux="rusanov", gamma=1.4):
+    """stencil: (ns, 5) nodes at offsets 1-ns/2 .. ns/2 of the face."""
     out = np.zeros(5)
-    rc = oracle().sfo_char_face_flux(_ptr(_arr(stencil).reshape(30)), _ptr(_arr(cofactor)),
-                                     SCHEMES[scheme], FLUXES[flux], gamma, _ptr(out))
+    sid = SCHEMES[scheme]
+    st = _arr(stencil).reshape(STENCIL[sid] * 5)
+    rc = oracle().sfo_char_face_flux_n(_ptr(st), _ptr(_arr(cofactor)), sid, FLUXES[flux],
+                                       gamma, _ptr(out))
     return rc, out

@@ -77,6 +77,102 @@ double sfo_teno5_core(const double v[5], unsigned *err)
     return (d0 * c0 * p0 + d1 * c1 * p1 + d2 * c2 * p2) / active;
 }
 
+/* Third-order WENO as the reference defines it: two three-point
+ * candidates (the WENO5 candidates S1, S2 shifted by one node), linear
+ * weights 1/3, 2/3, beta_k the squared central difference over the
+ * candidate, eps = 1e-6.  Follows WENO3::weno3_core,
+ * src/methods/numerics/convection/SF_WENO3.cpp:42-57 (v = offsets -1..2). */
+double sfo_weno3_core(const double v[4], unsigned *err)
+{
+    for (int s = 0; s < 4; ++s)
+        if (!isfinite(v[s])) { *err |= SFO_ERR_WENO; return NAN; }
+    double p0 = (-v[0] + 5.0 * v[1] + 2.0 * v[2]) / 6.0;
+    double p1 = (2.0 * v[1] + 5.0 * v[2] - v[3]) / 6.0;
+    const double d0 = 1.0 / 3.0, d1 = 2.0 / 3.0;
+    double b0 = (v[2] - v[0]) * (v[2] - v[0]);
+    double b1 = (v[3] - v[1]) * (v[3] - v[1]);
+    const double eps = 1e-6;
+    double a0 = d0 / ((eps + b0) * (eps + b0));
+    double a1 = d1 / ((eps + b1) * (eps + b1));
+    double sum = a0 + a1;
+    if (!isfinite(sum) || sum <= 0.0) { *err |= SFO_ERR_WENO; return NAN; }
+    return (a0 * p0 + a1 * p1) / sum;
+}
+
+/* Smoothness of the cubic through four equally spaced values (x = 0..3):
+ * the integral over [a, b] of p'^2 + p''^2 + p'''^2, with the cubic in
+ * Newton form (forward differences d1, d2/2, d3/6).  Follows beta4,
+ * src/methods/numerics/convection/SF_WENO7.cpp:43-68, term by term. */
+static double w7_beta(const double v[4], double a, double b)
+{
+    double d1 = v[1] - v[0];
+    double d2 = (v[2] - 2.0 * v[1] + v[0]) * 0.5;
+    double d3 = (v[3] - 3.0 * v[2] + 3.0 * v[1] - v[0]) / 6.0;
+    double A = d1 - d2 + 2.0 * d3;
+    double B = 2.0 * d2 - 6.0 * d3;
+    double C = 3.0 * d3;
+    /* F(b) - F(a) for F = x, x^2, x^3/3, x^4/2, x^5/5 */
+    double i1 = b - a;
+    double i2 = b * b - a * a;
+    double i3 = b * b * b / 3.0 - a * a * a / 3.0;
+    double i4 = (b * b) * (b * b) / 2.0 - (a * a) * (a * a) / 2.0;
+    double i5 = b * b * b * b * b / 5.0 - a * a * a * a * a / 5.0;
+    double beta1 = A * A * i1 + A * B * i2 + (B * B + 2.0 * A * C) * i3
+                 + B * C * i4 + C * C * i5;
+    double D = 2.0 * d2 - 6.0 * d3;
+    double E = 6.0 * d3;
+    double beta2 = D * D * i1 + D * E * i2 + E * E * i3;
+    double beta3 = 36.0 * d3 * d3 * i1;
+    return beta1 + beta2 + beta3;
+}
+
+double sfo_weno7_smoothness(const double v[4], double a, double b)
+{
+    return w7_beta(v, a, b);
+}
+
+/* Seventh-order WENO-JS (Balsara and Shu 2000 candidates and linear
+ * weights 1/35, 12/35, 18/35, 4/35; eps = 1e-6; alpha = d/(eps+beta)^2).
+ * Follows WENO7::weno7_core,
+ * src/methods/numerics/convection/SF_WENO7.cpp:70-98 (v = offsets -3..3;
+ * the smoothness interval of candidate k is the cell of node 0 in the
+ * local coordinate of that candidate). */
+double sfo_weno7_core(const double v[7], unsigned *err)
+{
+    for (int s = 0; s < 7; ++s)
+        if (!isfinite(v[s])) { *err |= SFO_ERR_WENO; return NAN; }
+    double p0 = (-3.0 * v[0] + 13.0 * v[1] - 23.0 * v[2] + 25.0 * v[3]) / 12.0;
+    double p1 = (v[1] - 5.0 * v[2] + 13.0 * v[3] + 3.0 * v[4]) / 12.0;
+    double p2 = (-v[2] + 7.0 * v[3] + 7.0 * v[4] - v[5]) / 12.0;
+    double p3 = (3.0 * v[3] + 13.0 * v[4] - 5.0 * v[5] + v[6]) / 12.0;
+    const double d0 = 1.0 / 35.0, d1 = 12.0 / 35.0, d2 = 18.0 / 35.0, d3 = 4.0 / 35.0;
+    double b0 = w7_beta(v + 0, 2.5, 3.5);
+    double b1 = w7_beta(v + 1, 1.5, 2.5);
+    double b2 = w7_beta(v + 2, 0.5, 1.5);
+    double b3 = w7_beta(v + 3, -0.5, 0.5);
+    const double eps = 1e-6;
+    double a0 = d0 / ((eps + b0) * (eps + b0));
+    double a1 = d1 / ((eps + b1) * (eps + b1));
+    double a2 = d2 / ((eps + b2) * (eps + b2));
+    double a3 = d3 / ((eps + b3) * (eps + b3));
+    double sum = a0 + a1 + a2 + a3;
+    if (!isfinite(sum) || sum <= 0.0) { *err |= SFO_ERR_WENO; return NAN; }
+    return (a0 * p0 + a1 * p1 + a2 * p2 + a3 * p3) / sum;
+}
+
+/* builtInConvectionRecipe halo widths,
+ * src/core/config/types/SF_termRecipe.h:217-218; stencil offsets
+ * src/methods/numerics/structured/SF_stencil.h:12-17. */
+int sfo_stencil_size(int scheme)
+{
+    return scheme == SFO_WENO3 ? 4 : scheme == SFO_WENO7 ? 8 : 6;
+}
+
+int sfo_halo(int scheme)
+{
+    return scheme == SFO_WENO3 ? 2 : scheme == SFO_WENO7 ? 4 : 3;
+}
+
 /* ---------------------------------------------------------------------- */
 /* Thermodynamic closures                                                  */
 /* ---------------------------------------------------------------------- */
@@ -303,34 +399,241 @@ int sfo_hllc(const double qL[5], const double qR[5], const double n[3],
     return 0;
 }
 
-/* Flux::characteristicWENOFaceFlux<6> with the scalar-stencil mapping of
- * fillScalarWENOStencils<6>,
- * src/solver/discretization/convection/SF_WENO.h:79-98,185-245. */
-int sfo_char_face_flux(const double st[30], const double cof[3], int scheme,
-                       int flux, double gamma, double out[5])
+/* Steger-Warming flux-vector splitting of one state with the reference's
+ * smoothed eigenvalue split (|lambda| < 2 eps_sw, eps_sw = 0.01, an
+ * absolute threshold in velocity units).  Follows
+ * Riemann::splitStegerWarming,
+ * src/methods/numerics/convection/SF_riemann.cpp:15-67. */
+static int steger_split(const double Q[5], const double n[3], double gamma,
+                        double fp[5], double fm[5])
 {
+    double p;
+    if (legacy_pressure(Q, gamma, &p)) return -1;
+    double r = Q[0];
+    double invR = 1.0 / r;
+    double u = Q[1] * invR, v = Q[2] * invR, w = Q[3] * invR;
+    double q2 = u * u + v * v + w * w;
+    double a = sqrt(gamma * p * invR);
+    double un = u * n[0] + v * n[1] + w * n[2];
+    double lam[5] = {un, un, un, un + a, un - a};
+    double lp[5], lm[5];
+    const double eps_sw = 0.01;
+    for (int i = 0; i < 5; ++i) {
+        if (fabs(lam[i]) < 2.0 * eps_sw) {
+            lp[i] = 0.5 * (lam[i] * lam[i] / (4.0 * eps_sw) + lam[i] + eps_sw);
+            lm[i] = lam[i] - lp[i];
+        } else {
+            lp[i] = 0.5 * (lam[i] + fabs(lam[i]));
+            lm[i] = 0.5 * (lam[i] - fabs(lam[i]));
+        }
+    }
+    double common = r / (2.0 * gamma);
+    const double *ls[2] = {lp, lm};
+    double *out[2] = {fp, fm};
+    for (int side = 0; side < 2; ++side) {
+        const double *l = ls[side];
+        double *f = out[side];
+        double W1 = 2.0 * (gamma - 1.0) * l[0] + l[3] + l[4];
+        double W2 = l[3] - l[4];
+        double W3 = l[3] + l[4];
+        f[0] = common * W1;
+        f[1] = common * (u * W1 + a * n[0] * W2);
+        f[2] = common * (v * W1 + a * n[1] * W2);
+        f[3] = common * (w * W1 + a * n[2] * W2);
+        f[4] = common * (0.5 * q2 * W1 + a * un * W2 + (a * a / (gamma - 1.0)) * W3);
+    }
+    return 0;
+}
+
+/* StegerWarmingFlux::flux, src/methods/numerics/Flux/SF_sw.cpp:21-27:
+ * F+ of the left state plus F- of the right state. */
+int sfo_steger_warming(const double qL[5], const double qR[5], const double n[3],
+                       double gamma, double out[5])
+{
+    double fp[5], fm[5], unused[5];
+    if (steger_split(qL, n, gamma, fp, unused)) return -1;
+    if (steger_split(qR, n, gamma, unused, fm)) return -1;
+    for (int v = 0; v < 5; ++v) out[v] = fp[v] + fm[v];
+    return 0;
+}
+
+/* Local Lax-Friedrichs splitting F+- = (F(Q) +- lambda Q)/2 with
+ * lambda = |u_n| + a of the same state.  Follows
+ * Riemann::splitLaxFriedrichs,
+ * src/methods/numerics/convection/SF_riemann.cpp:125-154. */
+static int laxf_split(const double Q[5], const double n[3], double gamma,
+                      double fp[5], double fm[5])
+{
+    double p;
+    if (legacy_pressure(Q, gamma, &p)) return -1;
+    double r = Q[0];
+    double invR = 1.0 / r;
+    double u = Q[1] * invR, v = Q[2] * invR, w = Q[3] * invR;
+    double e = Q[4];
+    double a = sqrt(gamma * p * invR);
+    double un = u * n[0] + v * n[1] + w * n[2];
+    double lam = fabs(un) + a;
+    double F[5];
+    F[0] = r * un;
+    F[1] = Q[1] * un + p * n[0];
+    F[2] = Q[2] * un + p * n[1];
+    F[3] = Q[3] * un + p * n[2];
+    F[4] = (e + p) * un;
+    for (int i = 0; i < 5; ++i) {
+        fp[i] = 0.5 * (F[i] + lam * Q[i]);
+        fm[i] = 0.5 * (F[i] - lam * Q[i]);
+    }
+    return 0;
+}
+
+/* LaxFriedrichsFlux::flux, src/methods/numerics/Flux/SF_lxF.cpp:21-27. */
+int sfo_lax_friedrichs(const double qL[5], const double qR[5], const double n[3],
+                       double gamma, double out[5])
+{
+    double fp[5], fm[5], unused[5];
+    if (laxf_split(qL, n, gamma, fp, unused)) return -1;
+    if (laxf_split(qR, n, gamma, unused, fm)) return -1;
+    for (int v = 0; v < 5; ++v) out[v] = fp[v] + fm[v];
+    return 0;
+}
+
+/* physicalEulerFlux, src/methods/numerics/Flux/SF_face.h:112-128
+ * (velocities by division, legacy pressure). */
+static int euler_flux(const double q[5], const double n[3], double gamma,
+                      double f[5])
+{
+    double p;
+    if (legacy_pressure(q, gamma, &p)) return -1;
+    const double rho = q[0];
+    const double u = q[1] / rho, v = q[2] / rho, w = q[3] / rho;
+    const double un = u * n[0] + v * n[1] + w * n[2];
+    f[0] = rho * un;
+    f[1] = rho * u * un + p * n[0];
+    f[2] = rho * v * un + p * n[1];
+    f[3] = rho * w * un + p * n[2];
+    f[4] = (q[4] + p) * un;
+    return 0;
+}
+
+/* Roe flux with Harten's entropy fix (delta = 0.1 a~) on every wave,
+ * F = (F_L + F_R - R |Lambda| L (Q_R - Q_L)) / 2 with L, R of
+ * buildCharacteristicMatrix at the Roe-averaged state.  Follows
+ * RoeFlux::flux, src/methods/numerics/Flux/SF_roe.cpp:20-111, including
+ * two of its choices: the eigenvectors are built at the state
+ * (rho~, rho~ u~, rho~ h~), whose energy slot holds rho~ h~, so their sound
+ * speed is sqrt(gamma) a~ while the wave speeds use a~; and the speeds come
+ * in the order (u_n, u_n, u_n, u_n + a~, u_n - a~) against the columns of
+ * R, whose fourth column is the u_n - c wave (SF_riemann.cpp:117-122). */
+int sfo_roe(const double qL[5], const double qR[5], const double n[3],
+            double gamma, double out[5])
+{
+    const double gm1 = gamma - 1.0;
+    double pL, pR;
+    if (legacy_pressure(qL, gamma, &pL) || legacy_pressure(qR, gamma, &pR)) return -1;
+    const double invRL = 1.0 / qL[0];
+    const double uL = qL[1] * invRL, vL = qL[2] * invRL, wL = qL[3] * invRL;
+    const double hL = (qL[4] + pL) * invRL;
+    const double invRR = 1.0 / qR[0];
+    const double uR = qR[1] * invRR, vR = qR[2] * invRR, wR = qR[3] * invRR;
+    const double hR = (qR[4] + pR) * invRR;
+    const double sqrtRL = sqrt(qL[0]), sqrtRR = sqrt(qR[0]);
+    const double sumSqrt = sqrtRL + sqrtRR;
+    const double uRoe = (sqrtRL * uL + sqrtRR * uR) / sumSqrt;
+    const double vRoe = (sqrtRL * vL + sqrtRR * vR) / sumSqrt;
+    const double wRoe = (sqrtRL * wL + sqrtRR * wR) / sumSqrt;
+    const double hRoe = (sqrtRL * hL + sqrtRR * hR) / sumSqrt;
+    const double q2Roe = uRoe * uRoe + vRoe * vRoe + wRoe * wRoe;
+    const double a2Roe = gm1 * (hRoe - 0.5 * q2Roe);
+    if (!isfinite(a2Roe) || a2Roe <= 0.0) return -1;
+    const double aRoe = sqrt(a2Roe);
+    const double unRoe = uRoe * n[0] + vRoe * n[1] + wRoe * n[2];
+    const double lambda[5] = {unRoe, unRoe, unRoe, unRoe + aRoe, unRoe - aRoe};
+    double absLambda[5];
+    const double entropyEps = 0.1 * aRoe;
+    for (int m = 0; m < 5; ++m) {
+        const double value = fabs(lambda[m]);
+        absLambda[m] = value < 2.0 * entropyEps
+            ? 0.5 * (value * value / (2.0 * entropyEps) + entropyEps) : value;
+    }
+    double dQ[5];
+    for (int v = 0; v < 5; ++v) dQ[v] = qR[v] - qL[v];
+    const double rhoRoe = sqrtRL * sqrtRR;
+    double qRoe[5];
+    qRoe[0] = rhoRoe;
+    qRoe[1] = rhoRoe * uRoe;
+    qRoe[2] = rhoRoe * vRoe;
+    qRoe[3] = rhoRoe * wRoe;
+    qRoe[4] = rhoRoe * (0.5 * q2Roe + a2Roe / gm1);
+    double L[25], R[25];
+    if (sfo_char_matrix(qRoe, n, gamma, L, R)) return -1;
+    double alpha[5];
+    sfo_gemv5(L, dQ, alpha);
+    double dis[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int wave = 0; wave < 5; ++wave) {
+        const double strength = absLambda[wave] * alpha[wave];
+        for (int v = 0; v < 5; ++v) dis[v] += R[v * 5 + wave] * strength;
+    }
+    double fL[5], fR[5];
+    if (euler_flux(qL, n, gamma, fL) || euler_flux(qR, n, gamma, fR)) return -1;
+    for (int v = 0; v < 5; ++v) out[v] = 0.5 * (fL[v] + fR[v] - dis[v]);
+    return 0;
+}
+
+int sfo_face_flux(int flux, const double qL[5], const double qR[5], const double n[3],
+                  double gamma, double out[5])
+{
+    switch (flux) {
+    case SFO_RUSANOV: return sfo_rusanov(qL, qR, n, gamma, out);
+    case SFO_HLLC: return sfo_hllc(qL, qR, n, gamma, out);
+    case SFO_STEGER: return sfo_steger_warming(qL, qR, n, gamma, out);
+    case SFO_LAXF: return sfo_lax_friedrichs(qL, qR, n, gamma, out);
+    case SFO_ROE: return sfo_roe(qL, qR, n, gamma, out);
+    }
+    return -1;
+}
+
+/* Flux::characteristicWENOFaceFlux<NS> with the scalar-stencil mapping of
+ * fillScalarWENOStencils<NS> (left: nodes 0..w-1, right: nodes NS-1..NS-w,
+ * w = 4, 5, 7 for NS = 4, 6, 8),
+ * src/solver/discretization/convection/SF_WENO.h:64-98,185-245. */
+int sfo_char_face_flux_n(const double *st, const double cof[3], int scheme,
+                         int flux, double gamma, double out[5])
+{
+    const int ns = sfo_stencil_size(scheme);
+    const int width = ns == 4 ? 4 : ns == 6 ? 5 : 7;
     const double smag = sqrt(cof[0] * cof[0] + cof[1] * cof[1] + cof[2] * cof[2]);
     if (!isfinite(smag) || smag <= 1.0e-300) return SFO_ERR_ARG;
     const double n[3] = {cof[0] / smag, cof[1] / smag, cof[2] / smag};
     const double area = smag;
+    const int cl = ns / 2 - 1;
     double qa[5];
-    for (int v = 0; v < 5; ++v) qa[v] = 0.5 * (st[2 * 5 + v] + st[3 * 5 + v]);
+    for (int v = 0; v < 5; ++v) qa[v] = 0.5 * (st[cl * 5 + v] + st[(cl + 1) * 5 + v]);
     double L[25], R[25];
     if (sfo_char_matrix(qa, n, gamma, L, R)) return SFO_ERR_CHAR;
-    double ch[6][5];
-    for (int s = 0; s < 6; ++s) sfo_gemv5(L, st + 5 * s, ch[s]);
+    double ch[8][5];
+    for (int s = 0; s < ns; ++s) sfo_gemv5(L, st + 5 * s, ch[s]);
     double wL[5], wR[5];
     unsigned err = 0;
     for (int v = 0; v < 5; ++v) {
-        double left[5], right[5];
-        for (int s = 0; s < 5; ++s) {
+        double left[8], right[8];
+        for (int s = 0; s < width; ++s) {
             left[s] = ch[s][v];
-            right[s] = ch[5 - s][v];
+            right[s] = ch[ns - 1 - s][v];
         }
-        if (scheme == SFO_TENO5) {
+        switch (scheme) {
+        case SFO_TENO5:
             wL[v] = sfo_teno5_core(left, &err);
             wR[v] = sfo_teno5_core(right, &err);
-        } else {
+            break;
+        case SFO_WENO3:
+            wL[v] = sfo_weno3_core(left, &err);
+            wR[v] = sfo_weno3_core(right, &err);
+            break;
+        case SFO_WENO7:
+            wL[v] = sfo_weno7_core(left, &err);
+            wR[v] = sfo_weno7_core(right, &err);
+            break;
+        default:
             wL[v] = sfo_weno5_core(left, &err);
             wR[v] = sfo_weno5_core(right, &err);
         }
@@ -340,11 +643,16 @@ int sfo_char_face_flux(const double st[30], const double cof[3], int scheme,
     double qL[5], qR[5], f[5];
     sfo_gemv5(R, wL, qL);
     sfo_gemv5(R, wR, qR);
-    int rc = flux == SFO_HLLC ? sfo_hllc(qL, qR, n, gamma, f)
-                              : sfo_rusanov(qL, qR, n, gamma, f);
-    if (rc) return SFO_ERR_FLUX;
+    if (sfo_face_flux(flux, qL, qR, n, gamma, f)) return SFO_ERR_FLUX;
     for (int v = 0; v < 5; ++v) out[v] = f[v] * area;
     return 0;
+}
+
+int sfo_char_face_flux(const double st[30], const double cof[3], int scheme,
+                       int flux, double gamma, double out[5])
+{
+    if (sfo_stencil_size(scheme) != 6) return SFO_ERR_ARG;
+    return sfo_char_face_flux_n(st, cof, scheme, flux, gamma, out);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -369,11 +677,22 @@ static size_t at(const sfo_grid *G, int i, int j, int k)
  * src/methods/numerics/structured/SF_iteration.h:37-58) unless it lies on
  * the boundary plane of a transmissive axis, which the boundary
  * applicator marks as a solver-boundary point (SF_applicator.h:162-187). */
+/* zeroGradient on the lower (side 0) or upper (side 1) end of axis d */
+static int zg_end(const sfo_grid *G, int d, int side)
+{
+    const int bc = G->desc.bc[d];
+    if (!G->desc.active[d]) return 0;
+    if (bc == SFO_TRANSMISSIVE) return 1;
+    if (bc == SFO_NONE_LO) return side == 1;
+    if (bc == SFO_NONE_HI) return side == 0;
+    return 0;
+}
+
 static int solved(const sfo_grid *G, const int p[3])
 {
     for (int d = 0; d < 3; ++d) {
-        if (!G->desc.active[d] || G->desc.bc[d] != SFO_TRANSMISSIVE) continue;
-        if (p[d] == G->g[d] || p[d] == G->g[d] + G->desc.n[d] - 1) return 0;
+        if (zg_end(G, d, 0) && p[d] == G->g[d]) return 0;
+        if (zg_end(G, d, 1) && p[d] == G->g[d] + G->desc.n[d] - 1) return 0;
     }
     return 1;
 }
@@ -386,12 +705,16 @@ sfo_grid *sfo_grid_create(const sfo_desc *desc)
         if (!desc->active[d] && desc->n[d] != 1) return NULL;
         if (!(desc->d[d] > 0.0)) return NULL;
     }
+    if (desc->scheme < SFO_WENO5 || desc->scheme > SFO_WENO7) return NULL;
+    for (int d = 0; d < 3; ++d)
+        if (desc->bc[d] < SFO_TRANSMISSIVE || desc->bc[d] > SFO_NONE) return NULL;
+    if (desc->flux < SFO_RUSANOV || desc->flux > SFO_ROE) return NULL;
     sfo_grid *G = (sfo_grid *)calloc(1, sizeof *G);
     if (!G) return NULL;
     G->desc = *desc;
     G->cells = 1;
     for (int d = 0; d < 3; ++d) {
-        G->g[d] = desc->active[d] ? 3 : 0;
+        G->g[d] = desc->active[d] ? sfo_halo(desc->scheme) : 0;
         G->m[d] = desc->n[d] + 2 * G->g[d];
         G->cells *= (size_t)G->m[d];
     }
@@ -439,6 +762,26 @@ int sfo_set_state(sfo_grid *G, const double *soa)
         double *q = G->q + 5 * at(G, p[0], p[1], p[2]);
         for (int v = 0; v < 5; ++v) q[v] = soa[v * N + u];
     }
+    /* an end without a condition: its ghost layers hold the end node's
+     * state from here on (nothing refills them) */
+    for (int d = 0; d < 3; ++d) {
+        if (!G->desc.active[d] || G->desc.bc[d] == SFO_TRANSMISSIVE
+            || G->desc.bc[d] == SFO_PERIODIC)
+            continue;
+        for (int side = 0; side < 2; ++side) {
+            if (zg_end(G, d, side)) continue;
+            FOR_NODES(G, p) {
+                const int end = side == 0 ? G->g[d] : G->g[d] + G->desc.n[d] - 1;
+                if (p[d] != end) continue;
+                for (int l = 1; l <= G->g[d]; ++l) {
+                    int gp[3] = {p[0], p[1], p[2]};
+                    gp[d] += side == 0 ? -l : l;
+                    memcpy(G->q + 5 * at(G, gp[0], gp[1], gp[2]),
+                           G->q + 5 * at(G, p[0], p[1], p[2]), 5 * sizeof(double));
+                }
+            }
+        }
+    }
     return 0;
 }
 
@@ -467,6 +810,7 @@ static void for_planes(sfo_grid *G, int d, plane_fn fn)
     int lo[3], hi[3];
     for (int a = 0; a < 3; ++a) { lo[a] = G->g[a]; hi[a] = G->g[a] + G->desc.n[a]; }
     for (int side = 0; side < 2; ++side) {
+        if (!zg_end(G, d, side)) continue;
         int p[3];
         for (p[e2] = lo[e2]; p[e2] < hi[e2]; ++p[e2])
             for (p[e1] = lo[e1]; p[e1] < hi[e1]; ++p[e1]) {
@@ -550,12 +894,8 @@ static void periodic_fill(sfo_grid *G, int d)
  * are visited axis by axis (x, y, z).  Periodic axes are filled last. */
 int sfo_apply_bc(sfo_grid *G)
 {
-    for (int d = 0; d < 3; ++d)
-        if (G->desc.active[d] && G->desc.bc[d] == SFO_TRANSMISSIVE)
-            for_planes(G, d, zg_copy4);
-    for (int d = 0; d < 3; ++d)
-        if (G->desc.active[d] && G->desc.bc[d] == SFO_TRANSMISSIVE)
-            for_planes(G, d, zg_energy);
+    for (int d = 0; d < 3; ++d) for_planes(G, d, zg_copy4);
+    for (int d = 0; d < 3; ++d) for_planes(G, d, zg_energy);
     for (int d = 0; d < 3; ++d)
         if (G->desc.active[d] && G->desc.bc[d] == SFO_PERIODIC)
             periodic_fill(G, d);
@@ -573,19 +913,20 @@ static int compute_fluxes(sfo_grid *G, int d)
     lo[d] = G->g[d] - 1;
     double cof[3] = {0.0, 0.0, 0.0};
     cof[d] = G->cof[d];
+    const int ns = sfo_stencil_size(G->desc.scheme);
     for (p[2] = lo[2]; p[2] < hi[2]; ++p[2])
         for (p[1] = lo[1]; p[1] < hi[1]; ++p[1])
             for (p[0] = lo[0]; p[0] < hi[0]; ++p[0]) {
-                double st[30];
-                for (int s = 0; s < 6; ++s) {
+                double st[40];
+                for (int s = 0; s < ns; ++s) {
                     int r[3] = {p[0], p[1], p[2]};
-                    r[d] += s - 2;
+                    r[d] += s + 1 - ns / 2;
                     memcpy(st + 5 * s, G->q + 5 * at(G, r[0], r[1], r[2]),
                            5 * sizeof(double));
                 }
                 double f[5];
-                int rc = sfo_char_face_flux(st, cof, G->desc.scheme,
-                                            G->desc.flux, G->desc.gamma, f);
+                int rc = sfo_char_face_flux_n(st, cof, G->desc.scheme,
+                                              G->desc.flux, G->desc.gamma, f);
                 if (rc) { G->err |= (unsigned)rc; return -1; }
                 for (int v = 0; v < 5; ++v) {
                     if (!isfinite(f[v])) { G->err |= SFO_ERR_FLUX; return -1; }

@@ -28,10 +28,15 @@
 extern "C" {
 #endif
 
-enum { SFO_WENO5 = 0, SFO_TENO5 = 1 };
-enum { SFO_RUSANOV = 0, SFO_HLLC = 1 };
+enum { SFO_WENO5 = 0, SFO_TENO5 = 1, SFO_WENO3 = 2, SFO_WENO7 = 3 };
+enum { SFO_RUSANOV = 0, SFO_HLLC = 1, SFO_STEGER = 2, SFO_LAXF = 3, SFO_ROE = 4 };
 enum { SFO_EULER = 0, SFO_SSPRK3 = 1, SFO_RK4 = 2 };
-enum { SFO_TRANSMISSIVE = 0, SFO_PERIODIC = 1 };
+/* per axis: zeroGradient on both ends, periodic, or an end that carries
+ * no condition (the case declares none for that set): its node is advanced
+ * like an interior node and its ghost layers keep the values they were
+ * given with the state (copies of the end node), never refilled. */
+enum { SFO_TRANSMISSIVE = 0, SFO_PERIODIC = 1, SFO_NONE_LO = 2, SFO_NONE_HI = 3,
+       SFO_NONE = 4 };
 
 /* error bits (the reference terminates the process on each of these) */
 enum {
@@ -47,6 +52,13 @@ enum {
 /* ---- point kernels ---------------------------------------------------- */
 double sfo_weno5_core(const double v[5], unsigned *err);
 double sfo_teno5_core(const double v[5], unsigned *err);
+double sfo_weno3_core(const double v[4], unsigned *err);
+double sfo_weno7_core(const double v[7], unsigned *err);
+/* the WENO7 smoothness indicator of one four-node candidate over [a, b] */
+double sfo_weno7_smoothness(const double v[4], double a, double b);
+/* stencil nodes of a scheme (4, 6 or 8) and its ghost depth (2, 3, 4) */
+int sfo_stencil_size(int scheme);
+int sfo_halo(int scheme);
 int sfo_char_matrix(const double q[5], const double n[3], double gamma,
                     double L[25], double R[25]);
 void sfo_gemv5(const double M[25], const double v[5], double out[5]);
@@ -54,12 +66,25 @@ int sfo_rusanov(const double qL[5], const double qR[5], const double n[3],
                 double gamma, double out[5]);
 int sfo_hllc(const double qL[5], const double qR[5], const double n[3],
              double gamma, double out[5]);
+int sfo_steger_warming(const double qL[5], const double qR[5], const double n[3],
+                       double gamma, double out[5]);
+int sfo_lax_friedrichs(const double qL[5], const double qR[5], const double n[3],
+                       double gamma, double out[5]);
+int sfo_roe(const double qL[5], const double qR[5], const double n[3],
+            double gamma, double out[5]);
+/* any of the five face fluxes by id */
+int sfo_face_flux(int flux, const double qL[5], const double qR[5], const double n[3],
+                  double gamma, double out[5]);
 int sfo_physical_flux(const double q[5], const double n[3], double gamma,
                       double out[5]);
 /* stencil is 6 nodes x 5 variables (offsets -2..+3 around face i+1/2);
  * cofactor is the face cofactor vector; out is the area-weighted flux. */
 int sfo_char_face_flux(const double stencil[30], const double cofactor[3],
                        int scheme, int flux, double gamma, double out[5]);
+/* the same for any scheme: sfo_stencil_size(scheme) nodes x 5 variables at
+ * offsets 1-ns/2 .. ns/2 around face i+1/2 */
+int sfo_char_face_flux_n(const double *stencil, const double cofactor[3],
+                         int scheme, int flux, double gamma, double out[5]);
 
 /* stage combination of one value, dq/dt = -r (see sfcpu.c) */
 double sfo_stage_combine(int time, int stage, double q, double q0,
@@ -72,7 +97,7 @@ typedef struct sfo_grid sfo_grid;
 typedef struct {
     int n[3];        /* unique nodes per axis (1 on an inactive axis)       */
     int active[3];   /* 1 = axis takes part in the discretisation           */
-    int bc[3];       /* SFO_TRANSMISSIVE or SFO_PERIODIC per axis            */
+    int bc[3];       /* SFO_TRANSMISSIVE, SFO_PERIODIC or SFO_NONE_* per axis */
     double d[3];     /* node spacing per axis                               */
     double gamma;
     int scheme, flux, time;

@@ -166,3 +166,101 @@ def test_teno5_last_bit_sensitivity_of_the_reference():
         worst[kw["scheme"]] = max(worst[kw["scheme"]], rel_errors(b, a)[1].max())
     assert worst["weno5"] <= 1e-13
     assert worst["teno5"] > 1e-9
+
+
+def test_weno3_weno7_cores_bit_exact():
+    import ctypes
+    P = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(41)
+    ref = ob.ref()
+    for _ in range(300):
+        v = rng.normal(size=7) * rng.choice([1e-3, 1.0, 1e3])
+        if rng.random() < 0.3:
+            v[rng.integers(2, 7):] += rng.choice([-1, 1]) * 10.0
+        a = np.ascontiguousarray(v)
+        assert ob.weno7(v)[0] == ref.sfref_weno7(a.ctypes.data_as(P))
+        b = np.ascontiguousarray(v[:4])
+        assert ob.weno3(v[:4])[0] == ref.sfref_weno3(b.ctypes.data_as(P))
+
+
+def test_splitting_fluxes_bit_exact():
+    import ctypes
+    P = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(43)
+    ref = ob.ref()
+    for _ in range(200):
+        qL = wl.conserved(np.array([rng.uniform(0.1, 3)]), *rng.uniform(-2, 2, 3),
+                          rng.uniform(0.1, 5))[:, 0].copy()
+        qR = wl.conserved(np.array([rng.uniform(0.1, 3)]), *rng.uniform(-2, 2, 3),
+                          rng.uniform(0.1, 5))[:, 0].copy()
+        if rng.random() < 0.2:
+            qR = qL.copy()
+            qR[1:4] *= 0.001        # slow flow: the Steger-Warming smoothing band
+        nv = rng.normal(size=3)
+        for n in (np.array([1.0, 0, 0]), np.array([0, 1.0, 0]), np.array([0, 0, 1.0]),
+                  nv / np.linalg.norm(nv)):
+            n = np.ascontiguousarray(n)
+            for flux in ("steger", "laxfriedrichs", "roe"):
+                out = np.zeros(5)
+                assert ref.sfref_split_flux(ob.FLUXES[flux], qL.ctypes.data_as(P),
+                                            qR.ctypes.data_as(P), n.ctypes.data_as(P), 1.4,
+                                            out.ctypes.data_as(P)) == 0
+                rc, o = ob.face_flux(flux, qL, qR, n)
+                assert rc == 0 and np.array_equal(o, out), flux
+
+
+@pytest.mark.parametrize("scheme", ["weno3", "weno5", "teno5", "weno7"])
+def test_every_scheme_and_flux_face_bit_exact(scheme):
+    import ctypes
+    P = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(47)
+    ref = ob.ref()
+    ns = ob.STENCIL[ob.SCHEMES[scheme]]
+    checked = 0
+    for _ in range(40):
+        rho = rng.uniform(0.2, 2, ns)
+        if rng.random() < 0.5:
+            rho[ns // 2:] *= 0.2
+        st = wl.conserved(rho, rng.uniform(-1, 1, ns), rng.uniform(-1, 1, ns),
+                          rng.uniform(-1, 1, ns), rng.uniform(0.5, 2, ns)).T.copy()
+        for axis in range(3):
+            cof = np.zeros(3)
+            cof[axis] = rng.uniform(0.01, 2)
+            for flux in ob.FLUXES:
+                rc, o = ob.char_face_flux(st, cof, scheme, flux)
+                if rc != 0:
+                    continue
+                out = np.zeros(5)
+                assert ref.sfref_char_face_flux_n(st.ctypes.data_as(P), cof.ctypes.data_as(P),
+                                                  ob.SCHEMES[scheme], ob.FLUXES[flux], 1.4,
+                                                  out.ctypes.data_as(P)) == 0
+                assert np.array_equal(o, out), (axis, scheme, flux)
+                checked += 1
+    assert checked > 400
+
+
+@pytest.mark.parametrize("n,active,bc", [
+    ((40, 1, 1), (1, 0, 0), ("transmissive",) * 3),
+    ((16, 14, 1), (1, 1, 0), ("periodic", "transmissive", "transmissive")),
+    ((16, 14, 1), (1, 1, 0), ("none_lo", "periodic", "transmissive")),
+    ((10, 9, 8), (1, 1, 1), ("transmissive", "periodic", "none"))])
+@pytest.mark.parametrize("scheme,flux,time", [
+    ("weno3", "steger", "ssprk3"), ("weno7", "roe", "rk4"), ("weno7", "laxfriedrichs", "euler"),
+    ("weno3", "hllc", "rk4"), ("weno5", "roe", "ssprk3"), ("teno5", "steger", "ssprk3"),
+    ("weno7", "rusanov", "ssprk3")])
+def test_new_schemes_and_fluxes_trajectory_bit_exact(n, active, bc, scheme, flux, time):
+    q = wl.smooth_field(n, seed=37)
+    desc = ob.make_desc(n, active, bc, (0.1, 0.12, 0.09), scheme=scheme, flux=flux, time=time)
+    a, da = ob.run(desc, q, 3, 0.4)
+    b, db = ob.ref_run(desc, q, 3, 0.4)
+    assert np.array_equal(da, db)
+    assert np.array_equal(a, b)
+
+
+def test_weno7_sod_baseline_case_bit_exact():
+    """The frozen WENO7 Sod case end to end (454 steps, output-aligned dt)."""
+    from tests.test_oracle_schemes import sod_weno7_desc, sod_weno7_state
+    q = sod_weno7_state()
+    a, da = ob.run_driver(sod_weno7_desc(), q, 0.0, 0.2, 0.5, write_time=0.05)
+    b, db = ob.ref_run_driver(sod_weno7_desc(), q, 0.0, 0.2, 0.5, write_time=0.05)
+    assert np.array_equal(da, db) and np.array_equal(a, b)
