@@ -31,14 +31,22 @@ extern "C" {
 
 /* ---- enumerations ------------------------------------------------------ */
 /* FDM::ConvectionScheme, core/config/types/SF_termRecipe.h:14 */
-enum { SF_SCHEME_WENO5 = 0, SF_SCHEME_TENO5 = 1 };
+enum { SF_SCHEME_WENO5 = 0, SF_SCHEME_TENO5 = 1, SF_SCHEME_WENO3 = 2, SF_SCHEME_WENO7 = 3 };
 /* FDM::FluxSplitter, SF_termRecipe.h:34-41; HLLC has no reference member */
-enum { SF_FLUX_RUSANOV = 0, SF_FLUX_HLLC = 1 };
+enum { SF_FLUX_RUSANOV = 0, SF_FLUX_HLLC = 1, SF_FLUX_STEGER_WARMING = 2,
+       SF_FLUX_LAX_FRIEDRICHS = 3, SF_FLUX_ROE = 4 };
 /* FDM::TimeRecipeId, core/config/types/SF_timeRecipe.h:10 */
 enum { SF_TIME_EULER = 0, SF_TIME_SSPRK3 = 1, SF_TIME_RK4 = 2 };
-/* zeroGradient (transmissive) boundary sets, or periodic (no reference
- * counterpart: BCType has no periodic kind) */
-enum { SF_BC_TRANSMISSIVE = 0, SF_BC_PERIODIC = 1 };
+/* Boundary kind of an axis: zeroGradient (transmissive) sets on both
+ * ends; periodic (no reference counterpart: BCType has no periodic kind);
+ * or an end whose set carries no density / velocity / pressure condition
+ * (lower, upper or both; the other end zeroGradient).  Such an end is not
+ * a solver-boundary set (SF_applicator.h:162-187): its node is advanced
+ * and enters the CFL scan like an interior node, and its ghost layers keep
+ * the values sf_state_upload gives them (copies of the end node), since no
+ * boundary law refills them. */
+enum { SF_BC_TRANSMISSIVE = 0, SF_BC_PERIODIC = 1, SF_BC_NONE_LO = 2, SF_BC_NONE_HI = 3,
+       SF_BC_NONE = 4 };
 
 /* error codes */
 enum {

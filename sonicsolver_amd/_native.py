@@ -14,10 +14,13 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "libsonic_amd.so"
 
-SCHEMES = {"weno5": 0, "teno5": 1}
-FLUXES = {"rusanov": 0, "hllc": 1}
+SCHEMES = {"weno5": 0, "teno5": 1, "weno3": 2, "weno7": 3}
+FLUXES = {"rusanov": 0, "hllc": 1, "steger": 2, "stegerwarming": 2, "laxfriedrichs": 3, "roe": 4}
 TIMES = {"forwardeuler": 0, "euler": 0, "ssprk3": 1, "classicalrk4": 2, "rk4": 2}
-BCS = {"transmissive": 0, "zerogradient": 0, "periodic": 1}
+# per axis; "none_lo" / "none_hi" / "none": the end carries no condition
+# (its node is advanced, its ghosts keep the uploaded end-node state)
+BCS = {"transmissive": 0, "zerogradient": 0, "periodic": 1, "none_lo": 2, "none_hi": 3,
+       "none": 4}
 
 FLAG_NAMES = {1: "state", 2: "face-average", 4: "teno-cutoff", 8: "flux", 16: "dt"}
 

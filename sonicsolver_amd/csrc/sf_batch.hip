@@ -702,9 +702,11 @@ int check_item(const sf_batch_item* it) {
     if (!(it->d[0] > 0.0) || (it->n[1] > 1 && !(it->d[1] > 0.0)))
         return bfail(SF_E_ARG, "node spacing must be positive");
     if (it->scheme != SF_SCHEME_WENO5 && it->scheme != SF_SCHEME_TENO5)
-        return bfail(SF_E_ARG, "unknown reconstruction scheme");
+        return bfail(SF_E_ARG, "the batch serves the WENO5 and TENO5 reconstructions "
+                               "(WENO3 and WENO7 run on the grid path)");
     if (it->flux != SF_FLUX_RUSANOV && it->flux != SF_FLUX_HLLC)
-        return bfail(SF_E_ARG, "unknown face flux");
+        return bfail(SF_E_ARG, "the batch serves the Rusanov and HLLC fluxes (Steger-Warming, "
+                               "Lax-Friedrichs and Roe run on the grid path)");
     if (!(it->cfl > 0.0) || !std::isfinite(it->cfl)) return bfail(SF_E_ARG, "invalid CFL number");
     return 0;
 }

@@ -2,6 +2,7 @@
 // device buffers, the per-stage launch sequence and the error handling.
 #include "sonic_amd.h"
 #include "sf_kernels.h"
+#include "sf_sweep.h"
 
 #include <hip/hip_runtime.h>
 
@@ -92,7 +93,7 @@ int check_desc(const sf_grid_desc* d) {
         if (d->active[a]) {
             ++nact;
             if (d->n[a] < 4) return fail(SF_E_ARG, "an active axis needs at least 4 nodes");
-            if (d->bc[a] != SF_BC_TRANSMISSIVE && d->bc[a] != SF_BC_PERIODIC)
+            if (d->bc[a] < SF_BC_TRANSMISSIVE || d->bc[a] > SF_BC_NONE)
                 return fail(SF_E_ARG, "unknown boundary kind");
         } else if (d->n[a] != 1) {
             return fail(SF_E_ARG, "an inactive axis must have exactly one node");
@@ -105,9 +106,9 @@ int check_desc(const sf_grid_desc* d) {
     if (d->active[2] && !d->active[1])
         return fail(SF_E_ARG, "a z-active grid must also be y-active");
     if (!(d->gamma > 1.0) || !std::isfinite(d->gamma)) return fail(SF_E_ARG, "gamma must exceed 1");
-    if (d->scheme != SF_SCHEME_WENO5 && d->scheme != SF_SCHEME_TENO5)
+    if (d->scheme < SF_SCHEME_WENO5 || d->scheme > SF_SCHEME_WENO7)
         return fail(SF_E_ARG, "unknown reconstruction scheme");
-    if (d->flux != SF_FLUX_RUSANOV && d->flux != SF_FLUX_HLLC)
+    if (d->flux < SF_FLUX_RUSANOV || d->flux > SF_FLUX_ROE)
         return fail(SF_E_ARG, "unknown face flux");
     if (d->time != SF_TIME_EULER && d->time != SF_TIME_SSPRK3 && d->time != SF_TIME_RK4)
         return fail(SF_E_ARG, "unknown time recipe");
@@ -150,15 +151,19 @@ int fill_ghosts(sf_grid* g, double* q) {
     auto plane = [&](int a) {
         return 2LL * G.n[(a + 1) % 3] * G.n[(a + 2) % 3];
     };
+    auto zero_gradient = [&](int a) {
+        return G.active[a] && (G.bc[a] == SF_BC_TRANSMISSIVE || G.bc[a] == SF_BC_NONE_LO ||
+                               G.bc[a] == SF_BC_NONE_HI);
+    };
     for (int a = 0; a < 3; ++a) {
-        if (!G.active[a] || G.bc[a] != 0) continue;
+        if (!zero_gradient(a)) continue;
         const unsigned nb = blocks_for(plane(a), 256);
         if (a == 0) k_bc_copy4<0><<<nb, 256, 0, g->stream>>>(G, q);
         if (a == 1) k_bc_copy4<1><<<nb, 256, 0, g->stream>>>(G, q);
         if (a == 2) k_bc_copy4<2><<<nb, 256, 0, g->stream>>>(G, q);
     }
     for (int a = 0; a < 3; ++a) {
-        if (!G.active[a] || G.bc[a] != 0) continue;
+        if (!zero_gradient(a)) continue;
         const unsigned nb = blocks_for(plane(a), 256);
         if (a == 0) k_bc_energy<0><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
         if (a == 1) k_bc_energy<1><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
@@ -176,53 +181,27 @@ int fill_ghosts(sf_grid* g, double* q) {
     return 0;
 }
 
-template <int SCHEME, int FLUX, int MODE>
-void launch_axis(sf_grid* g, int axis, const double* q, const StageArgs& A) {
-    const GridDev& G = g->G;
-    const int ev = prof_start(g, SF_KERNEL_SWEEP_X + axis);
-    if (axis == 0) {
-        // up to 16384 blocks, each looping over tiles of 127 faces
-        const long long faces = (long long)(G.n[0] + 1) * G.n[1] * G.n[2];
-        const long long tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
-        const unsigned bx = (unsigned)std::min<long long>(tiles, 16384);  // faces < 2^32 - 2^8
-        k_sweep_x<SCHEME, FLUX, MODE><<<bx, kXBlock, 0, g->stream>>>(G, q, g->S, A);
-        prof_stop(g, ev);
-        return;
-    }
-    const int other = axis == 1 ? G.n[2] : G.n[1];
-    dim3 grid(blocks_for(G.n[0], kMarchBlock), other, blocks_for(G.n[axis], kMarchSeg));
-    if (axis == 1)
-        k_sweep_march<1, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, g->stream>>>(G, q, g->S, A,
-                                                                                 kMarchSeg);
-    else
-        k_sweep_march<2, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, g->stream>>>(G, q, g->S, A,
-                                                                                 kMarchSeg);
-    prof_stop(g, ev);
-}
-
 // Sweeps run z, y, then x: the x sweep (one face per lane) applies the
 // stage update, because its register footprint is the smallest and the
 // update adds to it.  The flux differences are therefore summed as
 // (dz + dy) + dx rather than the reference's (dx + dy) + dz, which changes
 // the last bit at most.
-template <int SCHEME, int FLUX>
-void launch_sweeps(sf_grid* g, const double* q, const StageArgs& A) {
+int run_sweeps(sf_grid* g, const double* q, const StageArgs& A) {
+    SweepCall c{g->G, q, g->S, A, g->stream, kMarchSeg};
     int axes[3], na = 0;
     for (int a = 2; a >= 0; --a)
         if (g->G.active[a]) axes[na++] = a;
     for (int t = 0; t < na; ++t) {
-        if (t == na - 1) launch_axis<SCHEME, FLUX, kFinal>(g, axes[t], q, A);
-        else if (t == 0) launch_axis<SCHEME, FLUX, kCopyS>(g, axes[t], q, A);
-        else launch_axis<SCHEME, FLUX, kAccumS>(g, axes[t], q, A);
+        const int mode = (t == na - 1) ? kFinal : (t == 0) ? kCopyS : kAccumS;
+        const int ev = prof_start(g, SF_KERNEL_SWEEP_X + axes[t]);
+        switch (g->desc.scheme) {
+        case SF_SCHEME_TENO5: launch_sweep<1>(c, g->desc.flux, axes[t], mode); break;
+        case SF_SCHEME_WENO3: launch_sweep<2>(c, g->desc.flux, axes[t], mode); break;
+        case SF_SCHEME_WENO7: launch_sweep<3>(c, g->desc.flux, axes[t], mode); break;
+        default: launch_sweep<0>(c, g->desc.flux, axes[t], mode); break;
+        }
+        prof_stop(g, ev);
     }
-}
-
-int run_sweeps(sf_grid* g, const double* q, const StageArgs& A) {
-    const int s = g->desc.scheme, f = g->desc.flux;
-    if (s == 0 && f == 0) launch_sweeps<0, 0>(g, q, A);
-    else if (s == 0 && f == 1) launch_sweeps<0, 1>(g, q, A);
-    else if (s == 1 && f == 0) launch_sweeps<1, 0>(g, q, A);
-    else launch_sweeps<1, 1>(g, q, A);
     SF_HIP(hipGetLastError());
     return 0;
 }
@@ -380,7 +359,10 @@ int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
     GridDev& G = g->G;
     for (int a = 0; a < 3; ++a) {
         G.n[a] = desc->n[a];
-        G.g[a] = desc->active[a] ? 3 : 0;
+        // ghost depth = the scheme's halo (builtInConvectionRecipe,
+        // SF_termRecipe.h:217-218)
+        G.g[a] = desc->active[a] ? (desc->scheme == SF_SCHEME_WENO3 ? 2
+                                    : desc->scheme == SF_SCHEME_WENO7 ? 4 : 3) : 0;
         G.m[a] = G.n[a] + 2 * G.g[a];
         G.active[a] = desc->active[a];
         G.bc[a] = desc->bc[a];
@@ -469,9 +451,21 @@ int sf_grid_stream(sf_grid* g, void** stream) {
 }
 
 static int upload_from(sf_grid* g, const double* dsrc) {
-    const long long N = (long long)g->G.n[0] * g->G.n[1] * g->G.n[2];
+    const GridDev& G = g->G;
+    const long long N = (long long)G.n[0] * G.n[1] * G.n[2];
     const unsigned nb = std::min<unsigned>(blocks_for(N, 256), 65535u);
-    k_pack<<<nb, 256, 0, g->stream>>>(g->G, dsrc, g->buf[g->cur], 1);
+    k_pack<<<nb, 256, 0, g->stream>>>(G, dsrc, g->buf[g->cur], 1);
+    // ends without a condition: every state buffer's ghost layers there hold
+    // the uploaded end nodes from now on
+    for (int a = 0; a < 3; ++a) {
+        if (!G.active[a] || G.bc[a] < SF_BC_NONE_LO) continue;
+        const unsigned np = blocks_for(2LL * G.n[(a + 1) % 3] * G.n[(a + 2) % 3], 256);
+        for (int b = 0; b < 3; ++b) {
+            if (a == 0) k_bc_hold<0><<<np, 256, 0, g->stream>>>(G, g->buf[g->cur], g->buf[b]);
+            if (a == 1) k_bc_hold<1><<<np, 256, 0, g->stream>>>(G, g->buf[g->cur], g->buf[b]);
+            if (a == 2) k_bc_hold<2><<<np, 256, 0, g->stream>>>(G, g->buf[g->cur], g->buf[b]);
+        }
+    }
     SF_HIP(hipGetLastError());
     g->ghosts_valid = 0;
     g->dtmin_valid = 0;

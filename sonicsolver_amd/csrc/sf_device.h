@@ -41,11 +41,25 @@ __device__ __forceinline__ long long node_index(const GridDev& G, int i, int j, 
          + (i + G.g[0]);
 }
 
+// Boundary kind per axis (SF_BC_*): 0 zeroGradient on both ends,
+// 1 periodic, 2 / 3 / 4 no condition on the lower / upper / both ends (the
+// other end zeroGradient).  zg_end: the end (side 0 lower, 1 upper) carries
+// zeroGradient.
+__host__ __device__ __forceinline__ bool zg_end(const GridDev& G, int d, int side) {
+    const int b = G.bc[d];
+    return G.active[d] && (b == 0 || (b == 2 && side == 1) || (b == 3 && side == 0));
+}
+
+// A node is advanced unless it lies on a zeroGradient end, which the
+// boundary applicator marks as a solver-boundary point
+// (SF_applicator.h:162-187); an end without a condition is not marked.
 __device__ __forceinline__ bool solved_node(const GridDev& G, int i, int j, int k) {
     const int p[3] = {i, j, k};
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-        if (G.active[d] && G.bc[d] == 0 && (p[d] == 0 || p[d] == G.n[d] - 1)) return false;
+    for (int d = 0; d < 3; ++d) {
+        if (zg_end(G, d, 0) && p[d] == 0) return false;
+        if (zg_end(G, d, 1) && p[d] == G.n[d] - 1) return false;
+    }
     return true;
 }
 

@@ -1,6 +1,7 @@
 // sf_face.h -- per-face FP64 arithmetic of the explicit Euler hot path for
-// gfx950: characteristic WENO5-JS / TENO5 reconstruction on an axis-aligned
-// face followed by a Rusanov or HLLC flux.
+// gfx950: characteristic WENO3 / WENO5-JS / TENO5 / WENO7 reconstruction on
+// an axis-aligned face followed by a Rusanov, HLLC, Steger-Warming,
+// Lax-Friedrichs or Roe flux.
 //
 // The operations are those of the reference functions cited at each
 // definition (paths relative to the sonicSolver source tree).  On an
@@ -74,31 +75,6 @@ __device__ __forceinline__ double frsqrt(double z) {
 // sqrt(z) as z * rsqrt(z) (z > 0)
 __device__ __forceinline__ double fsqrt(double z) { return z * frsqrt(z); }
 
-// WENO5::weno5_core, src/methods/numerics/convection/SF_WENO5.cpp:42-62.
-// Same weights and candidates, evaluated with one division: with
-// e_k = (eps + beta_k)^2 and alpha_k = d_k / e_k,
-//   sum_k alpha_k p_k / sum_k alpha_k
-//     = sum_k d_k (6 p_k) E_k / (6 sum_k d_k E_k),   E_k = prod_{j != k} e_j,
-// and the candidates are kept as 6 p_k.  e_k >= 1e-12, and for the
-// characteristic variables (O(rho)) the products stay far from overflow.
-// Rounding differs from the seven-division form by a few ulps.
-__device__ __forceinline__ double weno5(double v0, double v1, double v2,
-                                        double v3, double v4) {
-    const double q0 = 2.0 * v0 - 7.0 * v1 + 11.0 * v2;
-    const double q1 = -v1 + 5.0 * v2 + 2.0 * v3;
-    const double q2 = 2.0 * v2 + 5.0 * v3 - v4;
-    const double b0 = 13.0 / 12.0 * sq(v0 - 2.0 * v1 + v2)
-                    + 0.25 * sq(v0 - 4.0 * v1 + 3.0 * v2);
-    const double b1 = 13.0 / 12.0 * sq(v1 - 2.0 * v2 + v3)
-                    + 0.25 * sq(v1 - v3);
-    const double b2 = 13.0 / 12.0 * sq(v2 - 2.0 * v3 + v4)
-                    + 0.25 * sq(3.0 * v2 - 4.0 * v3 + v4);
-    const double eps = 1e-6;
-    const double e0 = sq(eps + b0), e1 = sq(eps + b1), e2 = sq(eps + b2);
-    const double w0 = 0.1 * (e1 * e2), w1 = 0.6 * (e0 * e2), w2 = 0.3 * (e0 * e1);
-    return (w0 * q0 + w1 * q1 + w2 * q2) / (6.0 * (w0 + w1 + w2));
-}
-
 // x^6 to within a fraction of an ulp beyond the final rounding: the
 // products are carried with their exact rounding errors (FMA two-product),
 // so the result is the correctly rounded x^6 except within ~2^-100 of a
@@ -126,33 +102,6 @@ __device__ __forceinline__ bool teno_cut(double g, double gs) {
     const double P = 1.0e-5 * gs;
     const double E = __builtin_fma(1.0e-5, gs, -P);
     return g - P < E - 0x1p-70 * gs;
-}
-
-// TENO5::teno5_core, src/methods/numerics/convection/SF_TENO5.cpp:44-92
-// (C = 1, q = 6, CT = 1e-5).  The 1/6 of the candidates is folded into the
-// final division.
-__device__ __forceinline__ double teno5(double v0, double v1, double v2,
-                                        double v3, double v4, unsigned& flags) {
-    const double q0 = 2.0 * v0 - 7.0 * v1 + 11.0 * v2;
-    const double q1 = -v1 + 5.0 * v2 + 2.0 * v3;
-    const double q2 = 2.0 * v2 + 5.0 * v3 - v4;
-    const double b0 = 13.0 / 12.0 * sq(v0 - 2.0 * v1 + v2)
-                    + 0.25 * sq(v0 - 4.0 * v1 + 3.0 * v2);
-    const double b1 = 13.0 / 12.0 * sq(v1 - 2.0 * v2 + v3)
-                    + 0.25 * sq(v1 - v3);
-    const double b2 = 13.0 / 12.0 * sq(v2 - 2.0 * v3 + v4)
-                    + 0.25 * sq(3.0 * v2 - 4.0 * v3 + v4);
-    const double tau5 = fabs(b0 - b2);
-    const double g0 = pow6(1.0 + tau5 / (b0 + 1.0e-40));
-    const double g1 = pow6(1.0 + tau5 / (b1 + 1.0e-40));
-    const double g2 = pow6(1.0 + tau5 / (b2 + 1.0e-40));
-    const double gs = g0 + g1 + g2;
-    const double c0 = teno_cut(g0, gs) ? 0.0 : 1.0;
-    const double c1 = teno_cut(g1, gs) ? 0.0 : 1.0;
-    const double c2 = teno_cut(g2, gs) ? 0.0 : 1.0;
-    const double active = 0.1 * c0 + 0.6 * c1 + 0.3 * c2;
-    if (!(active > 0.0)) flags |= kFlagTeno;
-    return (0.1 * c0 * q0 + 0.6 * c1 * q1 + 0.3 * c2 * q2) / (6.0 * active);
 }
 
 // The left reconstruction at a face reads c0..c4 and the mirrored right one
@@ -235,18 +184,119 @@ __device__ __forceinline__ void teno5_pair(double c0, double c1, double c2, doub
     div_pair(nL, 6.0 * aL, nR, 6.0 * aR, left, right);
 }
 
-template <int SCHEME>
-__device__ __forceinline__ void recon_pair(const double ch[6], double& left, double& right,
-                                           unsigned& flags) {
-    if constexpr (SCHEME == 1) teno5_pair(ch[0], ch[1], ch[2], ch[3], ch[4], ch[5], left, right, flags);
-    else weno5_pair(ch[0], ch[1], ch[2], ch[3], ch[4], ch[5], left, right);
+// WENO3::weno3_core (SF_WENO3.cpp:42-57) on both sides of a face: the left
+// side reads c0..c3 and the right one c3..c0 (fillScalarWENOStencils<4>,
+// SF_WENO.h:79-98).  The right side's two candidates and smoothness
+// indicators are the left side's, exchanged, so each is formed once; the
+// weights are those of the reference, normalised with one division per
+// side (alpha_k = d_k / e_k multiplied through by e_A e_B) and the two
+// divisions share one reciprocal.
+__device__ __forceinline__ void weno3_pair(const double c[4], double& left, double& right) {
+    const double A = -c[0] + 5.0 * c[1] + 2.0 * c[2];   // 6 x left S0 = right S1
+    const double B = 2.0 * c[1] + 5.0 * c[2] - c[3];    // 6 x left S1 = right S0
+    const double eps = 1e-6;
+    const double eA = sq(eps + sq(c[2] - c[0]));
+    const double eB = sq(eps + sq(c[3] - c[1]));
+    const double d0 = 1.0 / 3.0, d1 = 2.0 / 3.0;
+    const double nL = d0 * eB * A + d1 * eA * B, nR = d0 * eA * B + d1 * eB * A;
+    const double sL = d0 * eB + d1 * eA, sR = d0 * eA + d1 * eB;
+    div_pair(nL, 6.0 * sL, nR, 6.0 * sR, left, right);
 }
 
+// Smoothness of one four-node WENO7 candidate over the cell [a, b] of its
+// local coordinate, as beta4 (SF_WENO7.cpp:43-68) forms it: the integral
+// of the squared first, second and third derivatives of the cubic with
+// forward differences d1, 2 d2, 6 d3.  b - a = 1 for every candidate, and
+// the second-derivative coefficients D = 2 d2 - 6 d3, E = 6 d3 of the
+// reference equal B and 2 C.
+struct W7Interval {
+    double i2, i3, i4, i5;   // F(b) - F(a) for x^2, x^3/3, x^4/2, x^5/5
+};
+
+constexpr W7Interval w7_interval(double a, double b) {
+    return {b * b - a * a, b * b * b / 3.0 - a * a * a / 3.0,
+            (b * b) * (b * b) / 2.0 - (a * a) * (a * a) / 2.0,
+            b * b * b * b * b / 5.0 - a * a * a * a * a / 5.0};
+}
+
+__device__ __forceinline__ double w7_beta(double v0, double v1, double v2, double v3,
+                                          const W7Interval& I) {
+    const double d1 = v1 - v0;
+    const double d2 = (v2 - 2.0 * v1 + v0) * 0.5;
+    const double d3 = (v3 - 3.0 * v2 + 3.0 * v1 - v0) * (1.0 / 6.0);
+    const double A = d1 - d2 + 2.0 * d3;
+    const double B = 2.0 * d2 - 6.0 * d3;
+    const double C = 3.0 * d3;
+    const double E = 2.0 * C;
+    const double beta1 = A * A + A * B * I.i2 + (B * B + 2.0 * A * C) * I.i3 + B * C * I.i4 +
+                         C * C * I.i5;
+    const double beta2 = B * B + B * E * I.i2 + E * E * I.i3;
+    const double beta3 = 36.0 * d3 * d3;
+    return beta1 + beta2 + beta3;
+}
+
+// WENO7::weno7_core (SF_WENO7.cpp:70-98) on both sides of a face: left
+// reads c0..c6, right c7..c1 (fillScalarWENOStencils<8>).  Three of the
+// four candidates of each side are candidates of the other side too (the
+// same four nodes in reverse, the same polynomial at the same face), so
+// five candidate values are formed; the smoothness intervals differ
+// between the sides (each side's target cell is its own upwind node), so
+// all eight indicators are formed.  Each side's weights are normalised
+// with one division (alpha_k = d_k / e_k multiplied through by the
+// product of the four e_j) and the two divisions share one reciprocal.
+__device__ __forceinline__ void weno7_side_weights(const double b[4], double w[4]) {
+    constexpr double d0 = 1.0 / 35.0, d1 = 12.0 / 35.0, d2 = 18.0 / 35.0, d3 = 4.0 / 35.0;
+    const double eps = 1e-6;
+    const double e0 = sq(eps + b[0]), e1 = sq(eps + b[1]);
+    const double e2 = sq(eps + b[2]), e3 = sq(eps + b[3]);
+    const double e01 = e0 * e1, e23 = e2 * e3;
+    w[0] = d0 * (e1 * e23);
+    w[1] = d1 * (e0 * e23);
+    w[2] = d2 * (e01 * e3);
+    w[3] = d3 * (e01 * e2);
+}
+
+__device__ __forceinline__ void weno7_pair(const double c[8], double& left, double& right) {
+    constexpr W7Interval I0 = w7_interval(2.5, 3.5), I1 = w7_interval(1.5, 2.5);
+    constexpr W7Interval I2 = w7_interval(0.5, 1.5), I3 = w7_interval(-0.5, 0.5);
+    // 12 x candidates: left S0, the shared ones (left S1 = right S3, left
+    // S2 = right S2, left S3 = right S1), right S0
+    const double L0 = -3.0 * c[0] + 13.0 * c[1] - 23.0 * c[2] + 25.0 * c[3];
+    const double P1 = c[1] - 5.0 * c[2] + 13.0 * c[3] + 3.0 * c[4];
+    const double P2 = -c[2] + 7.0 * c[3] + 7.0 * c[4] - c[5];
+    const double P3 = 3.0 * c[3] + 13.0 * c[4] - 5.0 * c[5] + c[6];
+    const double R0 = -3.0 * c[7] + 13.0 * c[6] - 23.0 * c[5] + 25.0 * c[4];
+    double bL[4], bR[4];
+    bL[0] = w7_beta(c[0], c[1], c[2], c[3], I0);
+    bL[1] = w7_beta(c[1], c[2], c[3], c[4], I1);
+    bL[2] = w7_beta(c[2], c[3], c[4], c[5], I2);
+    bL[3] = w7_beta(c[3], c[4], c[5], c[6], I3);
+    bR[0] = w7_beta(c[7], c[6], c[5], c[4], I0);
+    bR[1] = w7_beta(c[6], c[5], c[4], c[3], I1);
+    bR[2] = w7_beta(c[5], c[4], c[3], c[2], I2);
+    bR[3] = w7_beta(c[4], c[3], c[2], c[1], I3);
+    double wL[4], wR[4];
+    weno7_side_weights(bL, wL);
+    weno7_side_weights(bR, wR);
+    const double nL = wL[0] * L0 + wL[1] * P1 + wL[2] * P2 + wL[3] * P3;
+    const double nR = wR[0] * R0 + wR[1] * P3 + wR[2] * P2 + wR[3] * P1;
+    div_pair(nL, 12.0 * (wL[0] + wL[1] + wL[2] + wL[3]), nR,
+             12.0 * (wR[0] + wR[1] + wR[2] + wR[3]), left, right);
+}
+
+// Scheme ids (SF_SCHEME_*) and their stencils (SF_stencil.h:12-17):
+//   0 WENO5 and 1 TENO5 read 6 nodes, 2 WENO3 reads 4, 3 WENO7 reads 8.
+template <int SCHEME> struct Stencil { static constexpr int ns = 6; };
+template <> struct Stencil<2> { static constexpr int ns = 4; };
+template <> struct Stencil<3> { static constexpr int ns = 8; };
+
 template <int SCHEME>
-__device__ __forceinline__ double recon(double v0, double v1, double v2,
-                                        double v3, double v4, unsigned& flags) {
-    if constexpr (SCHEME == 1) return teno5(v0, v1, v2, v3, v4, flags);
-    else return weno5(v0, v1, v2, v3, v4);
+__device__ __forceinline__ void recon_pair(const double (&ch)[Stencil<SCHEME>::ns], double& left,
+                                           double& right, unsigned& flags) {
+    if constexpr (SCHEME == 1) teno5_pair(ch[0], ch[1], ch[2], ch[3], ch[4], ch[5], left, right, flags);
+    else if constexpr (SCHEME == 2) weno3_pair(ch, left, right);
+    else if constexpr (SCHEME == 3) weno7_pair(ch, left, right);
+    else weno5_pair(ch[0], ch[1], ch[2], ch[3], ch[4], ch[5], left, right);
 }
 
 // Closure of SingleFluidStateModel::close + PerfectGasEOS
@@ -347,87 +397,271 @@ __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
     for (int v = 0; v < 5; ++v) out[v] = f[v] + SK * (us[v] - q[v]);
 }
 
-// Flux::characteristicWENOFaceFlux<6> (SF_WENO.h:185-245) on a face of
-// axis AXIS whose cofactor is area * e_AXIS.  st[s][v] holds the stencil
-// nodes at offsets -2..+3 around the face.  Returns the area-weighted flux.
-template <int AXIS, int SCHEME, int FLUX>
-__device__ __forceinline__ void face_flux(const double st[6][5], double gamma,
-                                          double area, double out[5],
+// Pressure as Numerics::pressure forms it (SF_utility.h:24-29), the form
+// the flux splittings, the Roe flux and the characteristic matrix use.
+struct Legacy {
+    double inv_rho, u[3], p;
+    bool ok;
+};
+
+__device__ __forceinline__ Legacy legacy_state(const double q[5], double gamma) {
+    Legacy s;
+    s.inv_rho = frcp(q[0]);
+    const double ke = 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * s.inv_rho;
+    s.p = (gamma - 1.0) * (q[4] - ke);
+    s.u[0] = q[1] * s.inv_rho;
+    s.u[1] = q[2] * s.inv_rho;
+    s.u[2] = q[3] * s.inv_rho;
+    s.ok = q[0] > 0.0 && s.p > 0.0;
+    return s;
+}
+
+// One half of the Steger-Warming splitting of a state (POS: F+, else F-)
+// for n = e_ND, with the reference's smoothed split of eigenvalues below
+// 2 eps_sw = 0.02 (Riemann::splitStegerWarming, SF_riemann.cpp:15-67).
+// The three distinct eigenvalues u_n, u_n + a, u_n - a are split once each.
+template <int ND, bool POS>
+__device__ __forceinline__ void steger_half(const double q[5], double gamma, double f[5],
+                                            unsigned& flags) {
+    const Legacy s = legacy_state(q, gamma);
+    if (!s.ok) flags |= kFlagFlux;
+    const double a = fsqrt(gamma * s.p * s.inv_rho);
+    const double un = s.u[ND];
+    const double q2 = s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2];
+    auto split = [](double lam) {
+        const double eps_sw = 0.01;
+        double lp;
+        if (fabs(lam) < 2.0 * eps_sw) lp = 0.5 * (lam * lam * (1.0 / (4.0 * eps_sw)) + lam + eps_sw);
+        else lp = 0.5 * (lam + fabs(lam));
+        return POS ? lp : lam - lp;
+    };
+    const double l0 = split(un), l3 = split(un + a), l4 = split(un - a);
+    const double W1 = 2.0 * (gamma - 1.0) * l0 + l3 + l4;
+    const double W2 = l3 - l4;
+    const double W3 = l3 + l4;
+    const double common = q[0] * frcp(2.0 * gamma);
+    f[0] = common * W1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[1 + d] = common * ((d == ND) ? s.u[d] * W1 + a * W2 : s.u[d] * W1);
+    f[4] = common * (0.5 * q2 * W1 + a * un * W2 + (a * a) * frcp(gamma - 1.0) * W3);
+}
+
+// StegerWarmingFlux::flux (SF_sw.cpp:21-27): F+(qL) + F-(qR).
+template <int ND>
+__device__ __forceinline__ void steger_warming(const double qL[5], const double qR[5],
+                                               double gamma, double out[5], unsigned& flags) {
+    double fp[5], fm[5];
+    steger_half<ND, true>(qL, gamma, fp, flags);
+    steger_half<ND, false>(qR, gamma, fm, flags);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = fp[v] + fm[v];
+}
+
+// One half of the local Lax-Friedrichs splitting, (F(q) +- lambda q) / 2
+// with lambda = |u_n| + a of the state (Riemann::splitLaxFriedrichs,
+// SF_riemann.cpp:125-154).
+template <int ND, bool POS>
+__device__ __forceinline__ void laxf_half(const double q[5], double gamma, double f[5],
                                           unsigned& flags) {
-    using B = Basis<AXIS>;
-    constexpr int nd = B::nd, li = B::li, mi = B::mi;
-    constexpr double sl = B::sl, sm = B::sm;
-
-    // Face average (SF_WENO.h:207-211) and the eigenvector matrices of
-    // Riemann::buildCharacteristicMatrix (SF_riemann.cpp:69-123).
-    double qa[5];
+    const Legacy s = legacy_state(q, gamma);
+    if (!s.ok) flags |= kFlagFlux;
+    const double a = fsqrt(gamma * s.p * s.inv_rho);
+    const double un = s.u[ND];
+    const double lam = POS ? fabs(un) + a : -(fabs(un) + a);
+    double F[5];
+    F[0] = q[0] * un;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) qa[v] = 0.5 * (st[2][v] + st[3][v]);
-    const double invR = frcp(qa[0]);
-    const double ke = 0.5 * (qa[1] * qa[1] + qa[2] * qa[2] + qa[3] * qa[3]) * invR;
-    const double p = (gamma - 1.0) * (qa[4] - ke);
-    if (!(qa[0] > 0.0) || !(p > 0.0)) flags |= kFlagChar;
-    double u[3];
-    u[0] = qa[1] * invR;
-    u[1] = qa[2] * invR;
-    u[2] = qa[3] * invR;
-    const double q2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const double cz = gamma * p * invR;
-    const double rc = frsqrt(cz);
-    const double c = cz * rc;
-    const double H = (qa[4] + p) * invR;
-    const double un = u[nd];
-    const double ul = sl * u[li];
-    const double um = sm * u[mi];
-    const double b1 = (gamma - 1.0) * (rc * rc);
-    const double b2 = 0.5 * b1;
-    const double x1 = 0.5 * rc;
-
-    // Characteristic projections (gemv5 with L, SF_riemann.h:33-37).  With
-    // P = b2 (q2/2 rho - u.m + E) and X = x1 (un rho - m_n), the rows of L
-    // give w0 = rho - 2P (acoustic-free row), w3 = P + X, w4 = P - X, and
-    // the shear rows w1 = sl (m_l - u_l rho), w2 = sm (m_m - u_m rho).
-    const double hq2 = 0.5 * q2;
-    double ch[5][6];
+    for (int d = 0; d < 3; ++d) F[1 + d] = (d == ND) ? q[1 + d] * un + s.p : q[1 + d] * un;
+    F[4] = (q[4] + s.p) * un;
 #pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        const double r = st[s][0];
-        const double um_dot = u[0] * st[s][1] + u[1] * st[s][2] + u[2] * st[s][3];
-        const double P = b2 * (hq2 * r - um_dot + st[s][4]);
-        const double X = x1 * (un * r - st[s][1 + nd]);
-        ch[0][s] = r - 2.0 * P;
-        ch[1][s] = -ul * r + sl * st[s][1 + li];
-        ch[2][s] = -um * r + sm * st[s][1 + mi];
-        ch[3][s] = P + X;
-        ch[4][s] = P - X;
+    for (int v = 0; v < 5; ++v) f[v] = 0.5 * (F[v] + lam * q[v]);
+}
+
+// LaxFriedrichsFlux::flux (SF_lxF.cpp:21-27).
+template <int ND>
+__device__ __forceinline__ void lax_friedrichs(const double qL[5], const double qR[5],
+                                               double gamma, double out[5], unsigned& flags) {
+    double fp[5], fm[5];
+    laxf_half<ND, true>(qL, gamma, fp, flags);
+    laxf_half<ND, false>(qR, gamma, fm, flags);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = fp[v] + fm[v];
+}
+
+// The eigenvector matrices of Riemann::buildCharacteristicMatrix
+// (SF_riemann.cpp:69-123) at a state q for n = e_AXIS, in the factored
+// form used by the projections below.
+template <int AXIS>
+struct CharBasis {
+    double u[3], hq2, c, H, un, ul, um, b2, x1;
+
+    __device__ __forceinline__ CharBasis(const double q[5], double gamma, unsigned& flags) {
+        using B = Basis<AXIS>;
+        const double invR = frcp(q[0]);
+        const double ke = 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * invR;
+        const double p = (gamma - 1.0) * (q[4] - ke);
+        if (!(q[0] > 0.0) || !(p > 0.0)) flags |= kFlagChar;
+        u[0] = q[1] * invR;
+        u[1] = q[2] * invR;
+        u[2] = q[3] * invR;
+        hq2 = 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        const double cz = gamma * p * invR;
+        const double rc = frsqrt(cz);
+        c = cz * rc;
+        H = (q[4] + p) * invR;
+        un = u[B::nd];
+        ul = B::sl * u[B::li];
+        um = B::sm * u[B::mi];
+        b2 = 0.5 * ((gamma - 1.0) * (rc * rc));
+        x1 = 0.5 * rc;
     }
-    double wL[5], wR[5];
-#pragma unroll
-    for (int v = 0; v < 5; ++v) recon_pair<SCHEME>(ch[v], wL[v], wR[v], flags);
 
-    // Back to conservative variables with R (gemv5).  Every column of R
-    // carries u in the momentum rows except the acoustic +-c on the normal
-    // and the shear units, so q_m = u q_rho + (the rest).
-    double qL[5], qR[5];
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-        const double* w = side ? wR : wL;
-        double* q = side ? qR : qL;
+    // w = L q (gemv5, SF_riemann.h:33-37).  With P = b2 (q2/2 rho - u.m + E)
+    // and X = x1 (un rho - m_n), the rows of L give w0 = rho - 2P (the
+    // acoustic-free row), w3 = P + X, w4 = P - X, and the shear rows
+    // w1 = sl (m_l - u_l rho), w2 = sm (m_m - u_m rho).
+    __device__ __forceinline__ void project(const double q[5], double w[5]) const {
+        using B = Basis<AXIS>;
+        const double r = q[0];
+        const double um_dot = u[0] * q[1] + u[1] * q[2] + u[2] * q[3];
+        const double P = b2 * (hq2 * r - um_dot + q[4]);
+        const double X = x1 * (un * r - q[1 + B::nd]);
+        w[0] = r - 2.0 * P;
+        w[1] = -ul * r + B::sl * q[1 + B::li];
+        w[2] = -um * r + B::sm * q[1 + B::mi];
+        w[3] = P + X;
+        w[4] = P - X;
+    }
+
+    // q = R w.  Every column of R carries u in the momentum rows except the
+    // acoustic -+c on the normal and the shear units, so q_m = u q_rho +
+    // (the rest).
+    __device__ __forceinline__ void back(const double w[5], double q[5]) const {
+        using B = Basis<AXIS>;
         const double r = w[0] + w[3] + w[4];
         const double dw = w[4] - w[3];
         q[0] = r;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            if (d == nd) q[1 + d] = u[d] * r + c * dw;
-            else if (d == li) q[1 + d] = u[d] * r + sl * w[1];
-            else q[1 + d] = u[d] * r + sm * w[2];
+            if (d == B::nd) q[1 + d] = u[d] * r + c * dw;
+            else if (d == B::li) q[1 + d] = u[d] * r + B::sl * w[1];
+            else q[1 + d] = u[d] * r + B::sm * w[2];
         }
         q[4] = hq2 * w[0] + ul * w[1] + um * w[2] + H * (w[3] + w[4]) + (c * un) * dw;
     }
+};
 
+// RoeFlux::flux (SF_roe.cpp:20-111) for n = e_AXIS: Roe averages,
+// Harten's fix with delta = a~/10 on every wave, and the dissipation
+// R |Lambda| L (qR - qL) with R, L of the characteristic basis at the
+// state (rho~, rho~ u~, rho~ h~) -- the reference's energy slot holds
+// rho~ h~, so that basis has sound speed sqrt(gamma) a~ -- and the wave
+// speeds taken in the reference's order (u_n, u_n, u_n, u_n + a~,
+// u_n - a~) against the basis columns (the fourth is its u_n - c wave).
+template <int AXIS>
+__device__ __forceinline__ void roe(const double qL[5], const double qR[5], double gamma,
+                                    double out[5], unsigned& flags) {
+    constexpr int ND = Basis<AXIS>::nd;
+    const double gm1 = gamma - 1.0;
+    const Legacy l = legacy_state(qL, gamma), r = legacy_state(qR, gamma);
+    if (!l.ok || !r.ok) flags |= kFlagFlux;
+    const double hL = (qL[4] + l.p) * l.inv_rho, hR = (qR[4] + r.p) * r.inv_rho;
+    const double sL = fsqrt(qL[0]), sR = fsqrt(qR[0]);
+    const double inv = frcp(sL + sR);
+    double ut[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ut[d] = (sL * l.u[d] + sR * r.u[d]) * inv;
+    const double ht = (sL * hL + sR * hR) * inv;
+    const double q2t = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
+    const double a2 = gm1 * (ht - 0.5 * q2t);
+    if (!(a2 > 0.0)) flags |= kFlagFlux;
+    const double at = fsqrt(a2);
+    const double unt = ut[ND];
+    const double delta = 0.1 * at;
+    auto fix = [delta](double lam) {
+        const double v = fabs(lam);
+        return v < 2.0 * delta ? 0.5 * (v * v * frcp(2.0 * delta) + delta) : v;
+    };
+    const double lam0 = fix(unt), lam3 = fix(unt + at), lam4 = fix(unt - at);
+    const double rt = sL * sR;
+    double qt[5];
+    qt[0] = rt;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) qt[1 + d] = rt * ut[d];
+    qt[4] = rt * (0.5 * q2t + a2 * frcp(gm1));
+    const CharBasis<AXIS> cb(qt, gamma, flags);
+    double dq[5], al[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) dq[v] = qR[v] - qL[v];
+    cb.project(dq, al);
+    al[0] *= lam0;
+    al[1] *= lam0;
+    al[2] *= lam0;
+    al[3] *= lam3;
+    al[4] *= lam4;
+    double dis[5];
+    cb.back(al, dis);
+    // physicalEulerFlux (SF_face.h:112-128) of each side
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const double* q = side ? qR : qL;
+        const Legacy& s = side ? r : l;
+        const double un = s.u[ND];
+        double f[5];
+        f[0] = q[0] * un;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[1 + d] = (d == ND) ? q[0] * s.u[d] * un + s.p : q[0] * s.u[d] * un;
+        f[4] = (q[4] + s.p) * un;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) out[v] = side ? out[v] + f[v] : f[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) out[v] = 0.5 * (out[v] - dis[v]);
+}
+
+// Flux ids (SF_FLUX_*): 0 Rusanov, 1 HLLC, 2 Steger-Warming,
+// 3 Lax-Friedrichs, 4 Roe.
+template <int AXIS, int FLUX>
+__device__ __forceinline__ void riemann_flux(const double qL[5], const double qR[5],
+                                             double gamma, double out[5], unsigned& flags) {
+    constexpr int nd = Basis<AXIS>::nd;
+    if constexpr (FLUX == 1) hllc<nd>(qL, qR, gamma, out);
+    else if constexpr (FLUX == 2) steger_warming<nd>(qL, qR, gamma, out, flags);
+    else if constexpr (FLUX == 3) lax_friedrichs<nd>(qL, qR, gamma, out, flags);
+    else if constexpr (FLUX == 4) roe<AXIS>(qL, qR, gamma, out, flags);
+    else rusanov<nd>(qL, qR, gamma, out);
+}
+
+// Flux::characteristicWENOFaceFlux<NS> (SF_WENO.h:185-245) on a face of
+// axis AXIS whose cofactor is area * e_AXIS.  st[s][v] holds the stencil
+// nodes at offsets 1-NS/2 .. NS/2 around the face.  Returns the
+// area-weighted flux.
+template <int AXIS, int SCHEME, int FLUX>
+__device__ __forceinline__ void face_flux(const double (&st)[Stencil<SCHEME>::ns][5],
+                                          double gamma, double area, double out[5],
+                                          unsigned& flags) {
+    constexpr int NS = Stencil<SCHEME>::ns;
+    // face average (SF_WENO.h:207-211) and its characteristic basis
+    double qa[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) qa[v] = 0.5 * (st[NS / 2 - 1][v] + st[NS / 2][v]);
+    const CharBasis<AXIS> cb(qa, gamma, flags);
+    double ch[5][NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        double w[5];
+        cb.project(st[s], w);
+#pragma unroll
+        for (int v = 0; v < 5; ++v) ch[v][s] = w[v];
+    }
+    double wL[5], wR[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) recon_pair<SCHEME>(ch[v], wL[v], wR[v], flags);
+    double qL[5], qR[5];
+    cb.back(wL, qL);
+    cb.back(wR, qR);
     double f[5];
-    if constexpr (FLUX == 1) hllc<nd>(qL, qR, gamma, f);
-    else rusanov<nd>(qL, qR, gamma, f);
+    riemann_flux<AXIS, FLUX>(qL, qR, gamma, f, flags);
 #pragma unroll
     for (int v = 0; v < 5; ++v) out[v] = f[v] * area;
 }

@@ -1,0 +1,70 @@
+// sf_sweep.h -- launch of one sweep (one axis of one stage) of the grid
+// path.  Each reconstruction scheme's kernels are compiled in a translation
+// unit of their own (sf_sweep_<scheme>.hip instantiates launch_sweep for
+// its scheme id), so the twenty scheme x flux combinations build in
+// parallel.
+#pragma once
+
+#include "sf_sweep_kernels.h"
+
+#include <algorithm>
+
+namespace sfamd {
+
+struct SweepCall {
+    GridDev G;
+    const double* q;   // stage input state (ghosts filled)
+    double* S;         // flux-difference accumulator
+    StageArgs A;
+    hipStream_t stream;
+    int seg;           // nodes per march segment (y and z sweeps)
+};
+
+template <int SCHEME, int FLUX, int MODE>
+void launch_sweep_fm(const SweepCall& c, int axis) {
+    const GridDev& G = c.G;
+    if (axis == 0) {
+        // up to 16384 blocks, each looping over tiles of 127 faces
+        const long long faces = (long long)(G.n[0] + 1) * G.n[1] * G.n[2];
+        const long long tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
+        const unsigned bx = (unsigned)std::min<long long>(tiles, 16384);  // faces < 2^32 - 2^8
+        k_sweep_x<SCHEME, FLUX, MODE><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A);
+        return;
+    }
+    const unsigned other = (unsigned)(axis == 1 ? G.n[2] : G.n[1]);
+    const dim3 grid((unsigned)((G.n[0] + kMarchBlock - 1) / kMarchBlock), other,
+                    (unsigned)((G.n[axis] + c.seg - 1) / c.seg));
+    if (axis == 1)
+        k_sweep_march<1, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, c.stream>>>(G, c.q, c.S, c.A,
+                                                                                c.seg);
+    else
+        k_sweep_march<2, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, c.stream>>>(G, c.q, c.S, c.A,
+                                                                                c.seg);
+}
+
+template <int SCHEME, int FLUX>
+void launch_sweep_f(const SweepCall& c, int axis, int mode) {
+    if (mode == kCopyS) launch_sweep_fm<SCHEME, FLUX, kCopyS>(c, axis);
+    else if (mode == kAccumS) launch_sweep_fm<SCHEME, FLUX, kAccumS>(c, axis);
+    else launch_sweep_fm<SCHEME, FLUX, kFinal>(c, axis);
+}
+
+// One sweep of scheme SCHEME with face flux `flux` (SF_FLUX_*) along
+// `axis` in stage mode `mode` (kCopyS, kAccumS, kFinal).
+template <int SCHEME>
+void launch_sweep(const SweepCall& c, int flux, int axis, int mode) {
+    switch (flux) {
+    case 0: launch_sweep_f<SCHEME, 0>(c, axis, mode); break;
+    case 1: launch_sweep_f<SCHEME, 1>(c, axis, mode); break;
+    case 2: launch_sweep_f<SCHEME, 2>(c, axis, mode); break;
+    case 3: launch_sweep_f<SCHEME, 3>(c, axis, mode); break;
+    default: launch_sweep_f<SCHEME, 4>(c, axis, mode); break;
+    }
+}
+
+extern template void launch_sweep<0>(const SweepCall&, int, int, int);
+extern template void launch_sweep<1>(const SweepCall&, int, int, int);
+extern template void launch_sweep<2>(const SweepCall&, int, int, int);
+extern template void launch_sweep<3>(const SweepCall&, int, int, int);
+
+}  // namespace sfamd

@@ -1,0 +1,169 @@
+// sf_sweep_kernels.h -- the sweep kernels of the grid path: the x sweep
+// over flattened face tiles and the y / z march.  Templates only, so every
+// translation unit that instantiates a scheme can include it.
+#pragma once
+
+#include "sf_device.h"
+
+namespace sfamd {
+
+// ---------------------------------------------------------------------------
+// x sweep over the faces of all x lines numbered as one sequence: face
+// f = line * (n0 + 1) + (a + 1) lies between nodes a and a+1 of its line
+// (a = -1 .. n0-1).  Tile T is faces T*127 .. T*127+127: lane t evaluates
+// face T*127+t and lanes 1..127 form the difference for the node whose upper
+// face is theirs.  Tiles overlap by one face, so every node is updated by
+// exactly one lane, and no tile is left mostly idle at the end of a line.
+// Blocks loop over tiles (stride gridDim.x), so the CFL fold costs one
+// atomic per block.
+// ---------------------------------------------------------------------------
+constexpr int kXBlock = 128;
+
+// The final HLLC sweeps fit four waves per SIMD (128 VGPRs) without
+// spilling when asked to; the others stay at three (the final Rusanov sweeps
+// would spill at four).
+template <int SCHEME, int FLUX, int MODE>
+__global__ __launch_bounds__(kXBlock)
+__attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) ? 4 : 3, 8))) void k_sweep_x(GridDev G, const double* __restrict__ q,
+                                                     double* __restrict__ S, StageArgs A) {
+    __shared__ double sh[5][kXBlock];
+    __shared__ unsigned long long red[kXBlock / 64];
+    // face numbers fit 32 bits: the host rejects grids with 2^32 - 2^8
+    // faces or more along x
+    const int t = threadIdx.x;
+    const unsigned per_line = G.n[0] + 1;
+    const unsigned faces = per_line * G.n[1] * G.n[2];
+    const unsigned tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
+    unsigned flags = 0;
+    double dt_local = INFINITY;
+    for (unsigned T = blockIdx.x; T < tiles; T += gridDim.x) {
+        const unsigned f = T * (kXBlock - 1) + t;
+        const bool face = f < faces;
+        const unsigned line = face ? f / per_line : 0;
+        const int a = face ? (int)(f - line * per_line) - 1 : 0;  // lower node of the face
+        const int j = (int)(line % (unsigned)G.n[1]), k = (int)(line / (unsigned)G.n[1]);
+        if (face) {
+            constexpr int NS = Stencil<SCHEME>::ns;
+            const long long base = node_index(G, a, j, k);
+            double st[NS][5];
+#pragma unroll
+            for (int s = 0; s < NS; ++s)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + base + (s + 1 - NS / 2)];
+            double f5[5];
+            face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                if (!isfinite(f5[v])) flags |= kFlagFlux;
+                sh[v][t] = f5[v];
+            }
+        }
+        __syncthreads();
+        if (t >= 1 && face && a >= 0) {
+            const long long id = node_index(G, a, j, k);
+            double s5[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                const double d = sh[v][t] - sh[v][t - 1];
+                s5[v] = (MODE == kCopyS) ? d : S[v * G.P + id] + d;
+            }
+            if (MODE == kFinal) {
+                double dl = INFINITY;
+                if (solved_node(G, a, j, k)) stage_update(G, A, id, q, s5, flags, dl);
+                else carry_node(G, A, id, q);
+                dt_local = fmin(dt_local, dl);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
+            }
+        }
+        __syncthreads();  // sh is rewritten by the next tile
+    }
+    raise_flags(flags, A.flags);
+    if (MODE == kFinal && A.reduce_dt) block_min_atomic(dt_local, A.dtmin, red);
+}
+
+// ---------------------------------------------------------------------------
+// y / z sweep: lane = x index, blockIdx.y = the remaining transverse index,
+// blockIdx.z = segment of SEG nodes along the sweep axis.  Each lane marches
+// along its column; the NS-node stencil window lives in LDS (one private
+// slot set per lane, a ring indexed by `head`), so between faces only the
+// previous face flux stays in registers.  Lanes never read each other's
+// slots, so no barrier is needed.
+// ---------------------------------------------------------------------------
+constexpr int kMarchBlock = 64;
+
+template <int AXIS, int SCHEME, int FLUX, int MODE>
+__global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
+                                                             const double* __restrict__ q,
+                                                             double* __restrict__ S, StageArgs A,
+                                                             int seg) {
+    constexpr int NS = Stencil<SCHEME>::ns;
+    constexpr int H = NS / 2;   // face a+1/2 reads nodes a+1-H .. a+H
+    __shared__ double win[NS][5][kMarchBlock];
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x * kMarchBlock + lane;
+    const int o = blockIdx.y;
+    const int a0 = blockIdx.z * seg;
+    const int a1 = min(a0 + seg, G.n[AXIS]);
+    unsigned flags = 0;
+    double dt_local = INFINITY;
+    if (i < G.n[0]) {
+        const int j0 = (AXIS == 1) ? a0 : o;
+        const int k0 = (AXIS == 2) ? a0 : o;
+        const long long step = G.stride[AXIS];
+        // Preload nodes a0-H .. a0+H-2 into slots 1..NS-1; the march starts
+        // one node early (a = a0-1) so that face a0-1/2 is evaluated by the
+        // same instructions as every other face: a face's rounding then does
+        // not depend on where a segment starts.
+        long long id = node_index(G, i, j0, k0) - H * step;
+        for (int s = 0; s < NS - 1; ++s)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) win[s + 1][v][lane] = q[v * G.P + id + s * step];
+        id += (H - 1) * step;  // node a0 - 1
+        int head = 0;          // slot that receives the next node
+        double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma clang loop unroll(disable)
+        for (int a = a0 - 1; a < a1; ++a, id += step) {
+            // drop the oldest node, append node a+H
+#pragma unroll
+            for (int v = 0; v < 5; ++v) win[head][v][lane] = q[v * G.P + id + H * step];
+            head = (head == NS - 1) ? 0 : head + 1;
+            double st[NS][5];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int slot = (head + s >= NS) ? head + s - NS : head + s;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) st[s][v] = win[slot][v][lane];
+            }
+            double f[5];
+            face_flux<AXIS, SCHEME, FLUX>(st, G.gamma, G.area[AXIS], f, flags);
+            double s5[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                if (!isfinite(f[v])) flags |= kFlagFlux;
+                s5[v] = f[v] - fprev[v];
+                fprev[v] = f[v];
+            }
+            if (a < a0) continue;
+#pragma unroll
+            for (int v = 0; v < 5; ++v)
+                if (MODE != kCopyS) s5[v] = S[v * G.P + id] + s5[v];
+            if (MODE == kFinal) {
+                const int jj = (AXIS == 1) ? a : o;
+                const int kk = (AXIS == 2) ? a : o;
+                double dl = INFINITY;
+                if (solved_node(G, i, jj, kk)) stage_update(G, A, id, q, s5, flags, dl);
+                else carry_node(G, A, id, q);
+                dt_local = fmin(dt_local, dl);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
+            }
+        }
+    }
+    raise_flags(flags, A.flags);
+    if (MODE == kFinal && A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);
+}
+
+}  // namespace sfamd

@@ -1,0 +1,7 @@
+// sf_sweep_weno3.hip -- the grid-path sweep kernels of the WENO3 reconstruction
+// (scheme id 2) for all five face fluxes.
+#include "sf_sweep.h"
+
+namespace sfamd {
+template void launch_sweep<2>(const SweepCall&, int, int, int);
+}  // namespace sfamd

@@ -1,0 +1,7 @@
+// sf_sweep_weno7.hip -- the grid-path sweep kernels of the WENO7 reconstruction
+// (scheme id 3) for all five face fluxes.
+#include "sf_sweep.h"
+
+namespace sfamd {
+template void launch_sweep<3>(const SweepCall&, int, int, int);
+}  // namespace sfamd

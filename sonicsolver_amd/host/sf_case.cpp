@@ -176,6 +176,7 @@ struct sf_case {
     Mesh mesh;
     sf_grid_desc desc{};
     std::string convection, time;
+    int ghosts = -1;                             // nGhost of the mesh (-1: not given)
     double cfl = 1.0;
     double max_dt = std::numeric_limits<double>::max();
     double gamma = 1.4, gas_constant = 0.0;
@@ -247,8 +248,7 @@ void read_mesh(sf_case& c, const fs::path& dir) {
         if (!(c.mesh.hi[a] > c.mesh.lo[a])) reject(gen.rel, "the box must have positive extent");
         c.mesh.d[a] = (c.mesh.hi[a] - c.mesh.lo[a]) / (double)nc;
     }
-    if (const Node* ng = g.find("nGhost"))
-        if (ng->integer() < 3) reject(gen.rel, "nGhost must be at least 3 for the five-point schemes");
+    if (const Node* ng = g.find("nGhost")) c.ghosts = (int)ng->integer();
 }
 
 void read_fields(sf_case& c, const fs::path& dir) {
@@ -315,10 +315,14 @@ void read_fields(sf_case& c, const fs::path& dir) {
         }
         c.fields[fname] = fi;
     }
-    // Boundaries: every set of an active axis end must carry zeroGradient
-    // for rho, U and p; a set marked empty deactivates its axis.
+    // Boundaries: each end of an active axis carries zeroGradient for rho,
+    // U and p, or no condition for any of the three (the reference then
+    // neither marks it a solver boundary nor refills its ghosts,
+    // SF_applicator.h:51-83,162-187); a set marked empty deactivates its
+    // axis.
     int state[3][2];   // 0 unset, 1 zeroGradient, 2 empty
     std::memset(state, 0, sizeof state);
+    bool first_field = true;
     for (const char* fname : {"rho", "U", "p"}) {
         const std::string where = bnd.rel + " " + fname;
         const Node* f = bnd.body.find(fname);
@@ -363,14 +367,14 @@ void read_fields(sf_case& c, const fs::path& dir) {
         for (int a = 0; a < 3; ++a)
             for (int e = 0; e < 2; ++e) {
                 if (c.mesh.nodes[a] <= 1) continue;
-                if (!fstate[a][e])
-                    reject(where, std::string("the ") + (e ? "upper" : "lower") + " end of axis " +
-                                      "xyz"[a] + " has no boundary condition (this path needs "
-                                      "zeroGradient or empty there)");
-                if (state[a][e] && state[a][e] != fstate[a][e])
-                    reject(bnd.rel, "rho, U and p disagree on an axis end");
+                if (!first_field && state[a][e] != fstate[a][e])
+                    reject(bnd.rel, std::string("rho, U and p disagree on the ") +
+                                        (e ? "upper" : "lower") + " end of axis " + "xyz"[a] +
+                                        " (a condition for some of them and none for the "
+                                        "others is not served)");
                 state[a][e] = fstate[a][e];
             }
+        first_field = false;
     }
     for (int a = 0; a < 3; ++a) {
         if (c.mesh.nodes[a] <= 1) {
@@ -379,7 +383,16 @@ void read_fields(sf_case& c, const fs::path& dir) {
         }
         if ((state[a][0] == 2) != (state[a][1] == 2))
             reject(bnd.rel, std::string("axis ") + "xyz"[a] + " is empty on one end only");
-        c.desc.active[a] = state[a][0] == 1;
+        c.desc.active[a] = state[a][0] != 2;
+        if (!c.desc.active[a]) continue;
+        const bool lo = state[a][0] == 1, hi = state[a][1] == 1;
+        c.desc.bc[a] = lo && hi ? SF_BC_TRANSMISSIVE : hi ? SF_BC_NONE_LO
+                     : lo ? SF_BC_NONE_HI : SF_BC_NONE;
+        if (!(lo && hi))
+            c.notes.push_back(std::string("axis ") + "xyz"[a] + ": the " +
+                              (lo ? "upper" : hi ? "lower" : "lower and upper") +
+                              " end carries no rho/U/p condition (node advanced, ghosts hold "
+                              "the initial end state)");
     }
     if (!c.desc.active[0]) reject(bnd.rel, "the x axis must be active");
     if (c.desc.active[2] && !c.desc.active[1]) reject(bnd.rel, "z active with y empty is not supported");
@@ -460,7 +473,8 @@ void read_solvers(sf_case& c, const fs::path& dir) {
             reject(nu.rel, "terms.convection and convection.default disagree");
     } else {
         if (!have_scheme) reject(nu.rel, "no convection recipe (terms.convection or convection.default)");
-        recipe = builtInConvectionRecipe(scheme, have_flux ? flux : FluxSplitter::Rusanov);
+        // parseFluxSplitter("") is Steger-Warming (SF_numericsParser.h:77-80)
+        recipe = builtInConvectionRecipe(scheme, have_flux ? flux : parseFluxSplitter(""));
     }
     c.convection = recipe.id;
     std::string time_token;
@@ -618,9 +632,17 @@ int sf_case_open(const char* case_dir, sf_case** out) {
         read_models(*c, dir);
         for (int a = 0; a < 3; ++a) {
             c->desc.n[a] = c->desc.active[a] ? c->mesh.nodes[a] : 1;
-            c->desc.bc[a] = SF_BC_TRANSMISSIVE;
+            if (!c->desc.active[a]) c->desc.bc[a] = SF_BC_TRANSMISSIVE;
             c->desc.d[a] = c->mesh.d[a];
         }
+        // checkGhostDepth (SF_stencil.h:23-34): the mesh's ghost layers must
+        // cover the recipe's halo
+        const int halo = c->desc.scheme == SF_SCHEME_WENO3 ? 2
+                       : c->desc.scheme == SF_SCHEME_WENO7 ? 4 : 3;
+        if (c->ghosts >= 0 && c->ghosts < halo)
+            reject("mesh/generation.yaml", "nGhost " + std::to_string(c->ghosts) +
+                                               " is below the halo " + std::to_string(halo) +
+                                               " that " + c->convection + " needs");
         c->desc.gamma = c->gamma;
         c->desc.device = 0;
         if (c->control.end_step == 0 && !(c->control.end_time > c->control.start_time))

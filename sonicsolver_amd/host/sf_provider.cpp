@@ -32,9 +32,24 @@ std::string normalizeToken(std::string value) {
 namespace {
 
 const char* schemeName(ConvectionScheme s) {
-    return s == ConvectionScheme::TENO5 ? "teno5" : "weno5";
+    switch (s) {
+    case ConvectionScheme::TENO5: return "teno5";
+    case ConvectionScheme::WENO3: return "weno3";
+    case ConvectionScheme::WENO7: return "weno7";
+    default: return "weno5";
+    }
 }
-const char* fluxName(FluxSplitter f) { return f == FluxSplitter::HLLC ? "HLLC" : "Rusanov"; }
+
+// the flux part of toString(TermRecipeId) (SF_termRecipe.h:242-258)
+const char* fluxName(FluxSplitter f) {
+    switch (f) {
+    case FluxSplitter::HLLC: return "HLLC";
+    case FluxSplitter::StegerWarming: return "Steger";
+    case FluxSplitter::LaxFriedrichs: return "LaxFriedrichs";
+    case FluxSplitter::Roe: return "Roe";
+    default: return "Rusanov";
+    }
+}
 
 std::string servedList() {
     std::string out;
@@ -43,32 +58,42 @@ std::string servedList() {
 }
 
 // Names the reference registers (toString(TermRecipeId), SF_termRecipe.h:242-258)
-// that this provider does not serve.
+// that this provider does not serve: the Lax-Wendroff recipes and the
+// primitive upwind one.
 bool registeredElsewhere(const std::string& token) {
     static const char* schemes[] = {"weno3", "weno5", "teno5", "weno7"};
-    static const char* fluxes[] = {"Steger", "Rusanov", "LaxFriedrichs", "Roe", "LaxWendroff"};
-    for (const char* s : schemes)
-        for (const char* f : fluxes)
-            if (token == normalizeToken(std::string(s) + f)) return true;
+    for (const char* sc : schemes)
+        if (token == normalizeToken(std::string(sc) + "LaxWendroff")) return true;
     return token == "primitiveupwind1";
 }
 
 }  // namespace
 
+// The reference's registration order (TermRecipeId, SF_termRecipe.h:52-62:
+// schemes WENO3, WENO5, TENO5, WENO7; fluxes Steger, Rusanov,
+// LaxFriedrichs, Roe), then the HLLC recipes, which have no reference
+// counterpart.
 const std::vector<TermRecipe>& servedConvectionRecipes() {
-    static const std::vector<TermRecipe> recipes = {
-        builtInConvectionRecipe(ConvectionScheme::WENO5, FluxSplitter::Rusanov),
-        builtInConvectionRecipe(ConvectionScheme::TENO5, FluxSplitter::Rusanov),
-        builtInConvectionRecipe(ConvectionScheme::WENO5, FluxSplitter::HLLC),
-        builtInConvectionRecipe(ConvectionScheme::TENO5, FluxSplitter::HLLC),
-    };
+    static const std::vector<TermRecipe> recipes = [] {
+        std::vector<TermRecipe> r;
+        const ConvectionScheme schemes[] = {ConvectionScheme::WENO3, ConvectionScheme::WENO5,
+                                            ConvectionScheme::TENO5, ConvectionScheme::WENO7};
+        for (ConvectionScheme sc : schemes)
+            for (FluxSplitter f : {FluxSplitter::StegerWarming, FluxSplitter::Rusanov,
+                                   FluxSplitter::LaxFriedrichs, FluxSplitter::Roe})
+                r.push_back(builtInConvectionRecipe(sc, f));
+        for (ConvectionScheme sc : schemes)
+            r.push_back(builtInConvectionRecipe(sc, FluxSplitter::HLLC));
+        return r;
+    }();
     return recipes;
 }
 
-// builtInConvectionRecipe, SF_termRecipe.h:204-221 (halo 3 for the
-// five-point schemes).
+// builtInConvectionRecipe, SF_termRecipe.h:204-221: halo 2 for WENO3, 4 for
+// WENO7, 3 otherwise.
 TermRecipe builtInConvectionRecipe(ConvectionScheme scheme, FluxSplitter flux) {
-    return TermRecipe{std::string(schemeName(scheme)) + fluxName(flux), scheme, flux, 3};
+    const int halo = scheme == ConvectionScheme::WENO3 ? 2 : scheme == ConvectionScheme::WENO7 ? 4 : 3;
+    return TermRecipe{std::string(schemeName(scheme)) + fluxName(flux), scheme, flux, halo};
 }
 
 // resolveConvectionTermRecipe, SF_numericsParser.h:140-162.
@@ -89,26 +114,33 @@ ConvectionScheme parseConvectionScheme(const std::string& value) {
     const std::string t = normalizeToken(value);
     if (t.empty() || t == "weno5") return ConvectionScheme::WENO5;
     if (t == "teno5") return ConvectionScheme::TENO5;
-    if (t == "weno3" || t == "weno7" || t == "upwind1")
+    if (t == "weno3") return ConvectionScheme::WENO3;
+    if (t == "weno7") return ConvectionScheme::WENO7;
+    if (t == "upwind1")
         throw std::invalid_argument("Convection scheme '" + value +
-                                    "' is not served by the MI355X provider (WENO5, TENO5).");
+                                    "' is not served by the MI355X provider (WENO3, WENO5, "
+                                    "TENO5, WENO7).");
     throw std::invalid_argument("Unsupported convection scheme '" + value +
-                                "'. Supported schemes: WENO5, TENO5.");
+                                "'. Supported schemes: WENO3, WENO5, TENO5, WENO7.");
 }
 
 // parseFluxSplitter, SF_numericsParser.h:77-94 (empty selects
-// Steger-Warming there, which this provider does not serve).
+// Steger-Warming).
 FluxSplitter parseFluxSplitter(const std::string& value) {
     const std::string t = normalizeToken(value);
+    if (t.empty() || t == "stegerwarming" || t == "sw") return FluxSplitter::StegerWarming;
     if (t == "rusanov" || t == "localaxfriedrichs" || t == "llf") return FluxSplitter::Rusanov;
+    if (t == "laxfriedrichs" || t == "lf" || t == "lxf" || t == "laxf")
+        return FluxSplitter::LaxFriedrichs;
+    if (t == "roe") return FluxSplitter::Roe;
     if (t == "hllc") return FluxSplitter::HLLC;
-    if (t.empty() || t == "stegerwarming" || t == "sw" || t == "laxfriedrichs" || t == "lf" ||
-        t == "lxf" || t == "laxf" || t == "roe" || t == "laxwendroff" || t == "lw" ||
-        t == "upwindadvection")
+    if (t == "laxwendroff" || t == "lw" || t == "upwindadvection")
         throw std::invalid_argument("Flux method '" + value +
-                                    "' is not served by the MI355X provider (Rusanov, HLLC).");
+                                    "' is not served by the MI355X provider (StegerWarming, "
+                                    "Rusanov, LaxFriedrichs, Roe, HLLC).");
     throw std::invalid_argument("Unsupported inviscid flux method '" + value +
-                                "'. Supported methods: Rusanov, HLLC.");
+                                "'. Supported methods: StegerWarming, Rusanov, LaxFriedrichs, "
+                                "Roe, HLLC.");
 }
 
 // resolveTimeRecipe, SF_numericsParser.h:115-125 (empty selects classicalRK4).

@@ -30,14 +30,17 @@ namespace sfamd::host {
 std::string normalizeToken(std::string value);
 
 // ---- WHICH ------------------------------------------------------------------
-enum class ConvectionScheme { WENO5 = SF_SCHEME_WENO5, TENO5 = SF_SCHEME_TENO5 };
-enum class FluxSplitter { Rusanov = SF_FLUX_RUSANOV, HLLC = SF_FLUX_HLLC };
+enum class ConvectionScheme { WENO5 = SF_SCHEME_WENO5, TENO5 = SF_SCHEME_TENO5,
+                              WENO3 = SF_SCHEME_WENO3, WENO7 = SF_SCHEME_WENO7 };
+enum class FluxSplitter { Rusanov = SF_FLUX_RUSANOV, HLLC = SF_FLUX_HLLC,
+                          StegerWarming = SF_FLUX_STEGER_WARMING,
+                          LaxFriedrichs = SF_FLUX_LAX_FRIEDRICHS, Roe = SF_FLUX_ROE };
 
 struct TermRecipe {
     std::string id;             // canonical name, e.g. "weno5Rusanov"
     ConvectionScheme convection;
     FluxSplitter flux;
-    int haloWidth;              // 3 for the five-point schemes
+    int haloWidth;              // 2 WENO3, 3 WENO5 / TENO5, 4 WENO7
 };
 
 // Recipes served by this provider, in registration order.

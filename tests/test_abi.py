@@ -53,7 +53,9 @@ def test_invalid_descriptions_rejected_on_host():
         dict(n=(16, 1, 1), scheme=7),
         dict(n=(16, 1, 1), flux=5),
         dict(n=(16, 1, 1), time=9),
-        dict(n=(16, 1, 1), bc=(4, 0, 0)),
+        dict(n=(16, 1, 1), bc=(5, 0, 0)),
+        dict(n=(16, 1, 1), scheme=4),
+        dict(n=(16, 1, 1), bc=(-1, 0, 0)),
     ]
     for kw in bad:
         desc = _native.make_desc(**kw)

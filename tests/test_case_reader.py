@@ -56,7 +56,8 @@ def _edit(path, old, new):
 @pytest.mark.parametrize("file,old,new,message", [
     ("solvers/numerics.yaml", "weno5Rusanov", "weno7Rusanov", "weno7"),
     ("solvers/numerics.yaml", "terms:\n  convection: weno5Rusanov",
-     "convection:\n  default: TENO5\n  flux: StegerWarming", "StegerWarming"),
+     "convection:\n  default: TENO5\n  flux: LaxWendroff", "LaxWendroff"),
+    ("solvers/numerics.yaml", "weno5Rusanov", "weno5LaxWendroff", "not served"),
     ("fields/boundaries.yaml", "rho:\n  Left: zeroGradient\n", "rho:\n", "lower end of axis x"),
     ("fields/boundaries.yaml", "p:\n  Left: zeroGradient", "p:\n  Left: fixedValue",
      "fixedValue"),
@@ -113,10 +114,45 @@ def test_three_dimensional_and_one_dimensional(tmp_path):
 
 
 @pytest.mark.skipif(not REFERENCE_SOD.exists(), reason="reference case tree not present")
-def test_reference_sod_case_is_read_and_rejected_with_reason():
-    """The reference's own Sod case parses (every registry and document),
-    and is rejected for what this path does not serve: its density,
-    velocity and pressure have no condition on the Left set."""
-    with pytest.raises(SonicError) as e:
-        Case(REFERENCE_SOD)
-    assert "lower end of axis x" in str(e.value)
+def test_reference_sod_case_is_read():
+    """The reference's own Sod case (TENO5 + Steger-Warming + SSPRK3 in the
+    pair form) lowers onto the grid: its density, velocity and pressure
+    name no condition for the Left set, so that end carries none."""
+    with Case(REFERENCE_SOD) as c:
+        d = c.grid()
+        assert list(d.n) == [101, 21, 1] and list(d.active) == [1, 1, 0]
+        assert list(d.bc) == [2, 0, 0]
+        assert (d.scheme, d.flux, d.time) == (1, 2, 1)
+        assert c.numerics()[:3] == ("teno5Steger", "SSPRK3", 0.5)
+
+
+def test_recipes_beyond_weno5_and_ends_without_conditions(tmp_path):
+    case = _copy(tmp_path)
+    # WENO7 needs four ghost layers; with nGhost 4 every recipe id resolves
+    _edit(case / "mesh/generation.yaml", '"nGhost": 3', '"nGhost": 4')
+    for recipe, scheme, flux in (("weno7Rusanov", 3, 0), ("weno3Steger", 2, 2),
+                                 ("teno5LaxFriedrichs", 1, 3), ("weno7Roe", 3, 4),
+                                 ("weno3HLLC", 2, 1)):
+        text = (case / "solvers/numerics.yaml").read_text()
+        import re
+        (case / "solvers/numerics.yaml").write_text(
+            re.sub(r"convection: \w+", "convection: " + recipe, text))
+        with Case(case) as c:
+            d = c.grid()
+            assert (d.scheme, d.flux) == (scheme, flux), recipe
+            assert c.numerics()[0] == recipe
+    # the pair form without a flux selects Steger-Warming, as the reference does
+    text = (case / "solvers/numerics.yaml").read_text()
+    (case / "solvers/numerics.yaml").write_text(
+        re.sub(r"terms:\n  convection: \w+", "convection:\n  default: WENO5", text))
+    with Case(case) as c:
+        assert c.numerics()[0] == "weno5Steger"
+    # no Left condition for any of rho, U, p: the lower x end carries none
+    bnd = case / "fields/boundaries.yaml"
+    text = bnd.read_text()
+    text = text.replace("U:\n  Left: zeroGradient\n", "U:\n").replace(
+        "p:\n  Left: zeroGradient\n", "p:\n").replace("rho:\n  Left: zeroGradient\n", "rho:\n")
+    bnd.write_text(text)
+    with Case(case) as c:
+        assert list(c.grid().bc) == [2, 0, 0]
+        assert "no rho/U/p condition" in c.describe()

@@ -1,0 +1,163 @@
+"""The WENO3 / WENO7 reconstructions and the Steger-Warming, Lax-Friedrichs
+and Roe fluxes on the HIP path, against the oracle (relative L1 / Linf
+<= 1e-11 on the conserved variables), including ends that carry no
+boundary condition; and the reference's frozen WENO7 + Rusanov + forward
+Euler Sod run (tests/cases/sod_weno7_t0p2, baseline.json) through the
+case reader and the provider."""
+import json
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import bind as ob
+from sonicsolver_amd import Grid, Solver, make_desc, workloads as wl
+from sonicsolver_amd.case import Case
+from tests.parity import assert_close
+from tests.test_oracle_schemes import sod_weno7_desc, sod_weno7_state, sod_weno7_summary
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parent.parent
+SOD_W7 = ROOT / "tests" / "cases" / "sod_weno7_t0p2"
+GOLDEN = ROOT / "tests" / "golden" / "sod_weno7_t0p2" / "baseline.json"
+
+SCHEMES = ["weno3", "weno5", "teno5", "weno7"]
+FLUXES = ["rusanov", "hllc", "steger", "laxfriedrichs", "roe"]
+
+
+def _run_both(kw, q, steps, cfl):
+    print("case", kw, flush=True)
+    with Grid(make_desc(**kw)) as g:
+        g.upload(q)
+        dts = g.run(steps, cfl)
+        out = g.download()
+    ref, rdts = ob.run(ob.make_desc(**kw), q, steps, cfl)
+    np.testing.assert_allclose(dts, rdts, rtol=1e-12, atol=0)
+    return out, ref
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("flux", FLUXES)
+def test_scheme_flux_2d_smooth_with_an_open_end(scheme, flux):
+    n = (40, 34, 1)
+    q = wl.smooth_field(n, seed=19)
+    kw = dict(n=n, active=(1, 1, 0), bc=("none_lo", "periodic", "transmissive"),
+              d=(0.1, 0.12, 0.09), scheme=scheme, flux=flux, time="ssprk3")
+    out, ref = _run_both(kw, q, 10, 0.5)
+    assert_close(out, ref, what=f"{scheme}/{flux} 2D")
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("flux", FLUXES)
+def test_scheme_flux_sod_shock_tube(scheme, flux):
+    # a shock, a contact and a rarefaction, with flow at rest on both
+    # sides: the smoothed eigenvalue bands of Steger-Warming and Roe and the
+    # ENO weights at the discontinuities are all exercised
+    kw, q = wl.sod_1d(300, scheme=scheme, flux=flux, time="ssprk3")
+    out, ref = _run_both(kw, q, 40, 0.25)
+    assert_close(out, ref, what=f"{scheme}/{flux} Sod")
+
+
+def test_weno3_sod_failure_is_the_oracles():
+    # the reference's WENO3 drives the shock tube to a negative pressure at
+    # CFL 0.4 with RK4 (its validateEquationState would terminate the run);
+    # the device reports the same non-physical state instead of a result
+    from sonicsolver_amd import SonicError
+    kw, q = wl.sod_1d(300, scheme="weno3", flux="rusanov", time="rk4")
+    with pytest.raises(ob.OracleError) as eo:
+        ob.run(ob.make_desc(**kw), q, 40, 0.4)
+    assert eo.value.flags & 1
+    with Grid(make_desc(**kw)) as g:
+        g.upload(q)
+        with pytest.raises(SonicError) as ed:
+            g.run(40, 0.4)
+    assert ed.value.code == -3 and "flags 0x" in str(ed.value)
+
+
+@pytest.mark.parametrize("scheme,flux,time", [
+    ("weno3", "laxfriedrichs", "ssprk3"), ("weno7", "steger", "rk4"), ("weno7", "roe", "ssprk3"),
+    ("weno3", "roe", "euler"), ("teno5", "steger", "ssprk3"), ("weno5", "laxfriedrichs", "rk4")])
+def test_scheme_flux_3d_mixed_boundaries(scheme, flux, time):
+    n = (22, 18, 16)
+    q = wl.smooth_field(n, seed=23)
+    kw = dict(n=n, active=(1, 1, 1), bc=("periodic", "none", "none_hi"), d=(0.1, 0.12, 0.09),
+              scheme=scheme, flux=flux, time=time)
+    out, ref = _run_both(kw, q, 6, 0.4)
+    assert_close(out, ref, what=f"{scheme}/{flux} 3D")
+
+
+def test_weno7_sod_case_tree_lowers_as_the_oracle_describes_it():
+    with Case(SOD_W7) as c:
+        d = c.grid()
+        q0 = c.state()
+        assert list(d.n) == [101, 21, 1] and list(d.bc) == [2, 0, 0]
+        assert (d.scheme, d.flux, d.time) == (3, 0, 0)
+    np.testing.assert_array_equal(q0.reshape(5, -1), sod_weno7_state().reshape(5, -1))
+    ref = sod_weno7_desc()
+    assert list(ref.n) == list(d.n) and list(ref.bc) == list(d.bc)
+
+
+def test_weno7_sod_against_the_frozen_baseline():
+    """The whole run (454 steps to t = 0.2 with the output-aligned step) on
+    the device, judged by what the reference's regression script reads from
+    its final output, with the tolerances the oracle meets against the same
+    baseline: the left-state extrema to 1e-11, the integrals to 1e-10, the
+    extrema inside the forward-Euler oscillations at the right boundary to
+    1e-6 (they move with last-bit rounding; the reference's own FMA build
+    moves them as much), and the 7-digit dt log."""
+    base = json.loads(GOLDEN.read_text())
+    with Case(SOD_W7) as c:
+        geom = c.grid()
+        q0 = c.state()
+        conv, time, cfl, max_dt = c.numerics()
+        rc = c.run_control()
+    print("case sod_weno7_t0p2 via the provider", flush=True)
+    with Solver(geom, conv, time, cfl) as s:
+        s.upload(q0)
+        steps, t_end, outputs = s.run(rc["end_time"],
+                                      write_interval_time=rc["write_interval_time"])
+        out = s.download()
+        dts = s.dt_history()
+    assert steps == base["steps"] and len(dts) == steps
+    assert abs(t_end - 0.2) < 1e-15
+    assert [round(o[1], 12) for o in outputs] == [0.05, 0.1, 0.15, 0.2]
+    ref = np.array(base["dt_sequence"])
+    printed = np.array([float("%.7g" % x) for x in dts])
+    assert np.sum(printed == ref) >= 448
+    assert np.max(np.abs(dts - ref) / ref) < 1e-6
+    s_dev = sod_weno7_summary(out)
+    for name, k in (("rho", 1), ("p", 1), ("speed", 0)):
+        assert abs(s_dev["ranges"][name][k] - base["ranges"][name][k]) \
+            <= 1e-11 * abs(base["ranges"][name][k]), name
+    for name in ("rho", "p", "speed"):
+        for k in (0, 1):
+            assert abs(s_dev["ranges"][name][k] - base["ranges"][name][k]) \
+                <= 1e-6 * abs(base["ranges"][name][k]), name
+    for name in ("mass", "energy"):
+        assert abs(s_dev["integrals"][name] - base["integrals"][name]) \
+            < 1e-10 * base["integrals"][name], name
+    mx = base["integrals"]["momentum"][0]
+    assert abs(s_dev["integrals"]["momentum"][0] - mx) < 1e-10 * mx
+    # and the first 100 steps, before the oscillations grow, at the 1e-11 bar
+    a, _ = ob.run_driver(sod_weno7_desc(), q0, 0.0, 0.2, 0.5, write_time=0.05, end_step=100)
+    with Solver(geom, conv, time, cfl) as s:
+        s.upload(q0)
+        s.run(rc["end_time"], end_step=100, write_interval_time=rc["write_interval_time"])
+        early = s.download()
+    assert_close(early, a, what="sod weno7, 100 steps")
+
+
+def test_weno7_sod_case_writes_its_outputs(tmp_path):
+    with Case(SOD_W7) as c:
+        out, steps, t_end = c.run(output_dir=str(tmp_path))
+    assert steps == 454
+    files = sorted(p.name for p in tmp_path.glob("*.vts"))
+    job = "SodWENO7RusanovT0p2"
+    assert files == sorted(f"{job}_t{t}.vts" for t in ("0", "0.05", "0.1", "0.15", "0.2"))
+    text = (tmp_path / f"{job}_t0.2.vts").read_text()
+    assert 'WholeExtent="0 100 0 20 0 1"' in text
+    m = re.search(r'Name="rho"[^>]*>\n(.*?)\n\s*</DataArray>', text, re.S)
+    rho = np.array(m.group(1).split(), dtype=float).reshape(2, 21, 101)
+    np.testing.assert_array_equal(rho[1], out[0].reshape(21, 101))

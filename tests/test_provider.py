@@ -10,6 +10,14 @@ from sonicsolver_amd import (SonicError, describe_plan, recipe_from_pair, resolv
 @pytest.mark.parametrize("name,expected", [
     ("weno5Rusanov", (0, 0, 3)), ("teno5Rusanov", (1, 0, 3)),
     ("weno5HLLC", (0, 1, 3)), ("teno5HLLC", (1, 1, 3)),
+    # the reference's other registered ids (toString(TermRecipeId)), with
+    # the halo of each scheme (2 WENO3, 3 WENO5 / TENO5, 4 WENO7)
+    ("weno3Steger", (2, 2, 2)), ("weno3Rusanov", (2, 0, 2)), ("weno3LaxFriedrichs", (2, 3, 2)),
+    ("weno3Roe", (2, 4, 2)), ("weno5Steger", (0, 2, 3)), ("weno5LaxFriedrichs", (0, 3, 3)),
+    ("weno5Roe", (0, 4, 3)), ("teno5Steger", (1, 2, 3)), ("teno5LaxFriedrichs", (1, 3, 3)),
+    ("teno5Roe", (1, 4, 3)), ("weno7Steger", (3, 2, 4)), ("weno7Rusanov", (3, 0, 4)),
+    ("weno7LaxFriedrichs", (3, 3, 4)), ("weno7Roe", (3, 4, 4)),
+    ("weno3HLLC", (2, 1, 2)), ("weno7HLLC", (3, 1, 4)),
     # the reference's token normalisation: quotes, '_', '-', spaces, case
     ("WENO5_Rusanov", (0, 0, 3)), ('"teno5-hllc"', (1, 1, 3)), (" Weno5 HLLC ", (0, 1, 3)),
 ])
@@ -17,8 +25,8 @@ def test_served_recipes_resolve(name, expected):
     assert resolve_recipe(name) == expected
 
 
-@pytest.mark.parametrize("name", ["weno7Rusanov", "weno5Steger", "teno5Roe", "weno3LaxFriedrichs",
-                                  "weno5LaxWendroff", "primitiveUpwind1"])
+@pytest.mark.parametrize("name", ["weno3LaxWendroff", "weno5LaxWendroff", "teno5LaxWendroff",
+                                  "weno7LaxWendroff", "primitiveUpwind1"])
 def test_reference_recipes_not_served_are_rejected_explicitly(name):
     with pytest.raises(SonicError) as e:
         resolve_recipe(name)
@@ -35,7 +43,12 @@ def test_legacy_pair_form():
     assert recipe_from_pair("TENO5", "Rusanov") == (1, 0, 3)
     assert recipe_from_pair("", "llf") == (0, 0, 3)          # empty scheme selects WENO5
     assert recipe_from_pair("weno5", "HLLC") == (0, 1, 3)
-    for scheme, flux in (("weno5", ""), ("weno5", "StegerWarming"), ("weno7", "rusanov")):
+    assert recipe_from_pair("weno5", "") == (0, 2, 3)        # empty flux selects Steger-Warming
+    assert recipe_from_pair("weno7", "sw") == (3, 2, 4)
+    assert recipe_from_pair("WENO3", "lxf") == (2, 3, 2)
+    assert recipe_from_pair("teno5", "Roe") == (1, 4, 3)
+    for scheme, flux in (("weno5", "LaxWendroff"), ("upwind1", "rusanov"), ("weno9", "roe"),
+                         ("weno5", "godunov")):
         with pytest.raises(SonicError):
             recipe_from_pair(scheme, flux)
 
