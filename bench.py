@@ -209,6 +209,41 @@ def configs_1d_2d():
     return out
 
 
+def recipes_run(n: int, steps: int):
+    """Every registered convection recipe (WENO3 / WENO5 / TENO5 / WENO7 x
+    Steger-Warming / Rusanov / Lax-Friedrichs / Roe, plus HLLC) on the same
+    n^3 periodic Taylor-Green-type field with SSPRK3 at CFL 0.4: steps timed
+    after one warm-up step, the state resident on the device.  Reported as
+    cell updates per second and as FP64 rate against the per-recipe
+    operation count."""
+    from sonicsolver_amd import Grid, make_desc, workloads as wl, flops
+    kw, q = wl.tgv_3d(n)
+    out = {}
+    for scheme in ("weno3", "weno5", "teno5", "weno7"):
+        for flux in ("steger", "rusanov", "laxfriedrichs", "roe", "hllc"):
+            kw2 = dict(kw, scheme=scheme, flux=flux)
+            with Grid(make_desc(**kw2)) as g:
+                g.upload(q)
+                g.run(1, 0.4)
+                g.synchronize()
+                t0 = time.perf_counter()
+                g.run(steps, 0.4)
+                g.synchronize()
+                el = time.perf_counter() - t0
+                fl = g.flags()
+            per = flops.flops_per_cell_step(3, scheme, flux, "ssprk3")
+            tf = per * n ** 3 * steps / el / 1e12
+            out[f"{scheme}_{flux}"] = {"mcell_updates_per_s": round(n ** 3 * steps / el / 1e6, 1),
+                                       "ms_per_step": round(el / steps * 1e3, 3),
+                                       "flops_per_cell_step": per,
+                                       "achieved_tflops": round(tf, 2),
+                                       "fp64_frac": round(tf / FP64_PEAK_TFLOPS, 4),
+                                       "device_flags": fl}
+            log(f"recipe {scheme}/{flux}: {out[f'{scheme}_{flux}']['mcell_updates_per_s']} Mcell/s")
+    return {"workload": f"3D Euler {n}^3 periodic Taylor-Green-type field, SSPRK3, CFL 0.4, "
+                        f"FP64, {steps} timed steps per recipe", "recipes": out}
+
+
 def _serve_pass(reqs, rate, budget_gb, seed, keep):
     """One pass of the request stream through a fresh server: Poisson
     arrivals at `rate`/s submitted from a client thread; returns the
@@ -349,6 +384,8 @@ def main():
                     help="problems solved one case at a time for the plain comparison")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 1D shock tube and 2D vortex configurations")
+    ap.add_argument("--recipes", type=int, default=256,
+                    help="n of the n^3 grid every convection recipe is timed on (0 skips)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a device copy of known size (2 GiB read + 2 GiB written, "
                          "5 times) to calibrate HBM traffic counters, then exit")
@@ -476,6 +513,9 @@ def main():
     if not args.no_small:
         log("1D shock tube and 2D vortex configurations")
         line["single_grid"] = configs_1d_2d()
+    if args.recipes:
+        log(f"every convection recipe on {args.recipes}^3")
+        line["recipes"] = recipes_run(args.recipes, 3)
     if args.batched:
         line["batched"] = batched_run(args.batched, args.batched_steps, args.plain_sample)
     if args.served:

@@ -63,12 +63,13 @@ unsigned long long counted(void (*fn)()) {
     return Fl::count;
 }
 
-Fl g_stencil[30];
+Fl g_stencil[40];
 int g_scheme = 0, g_flux = 0;
 
 void set_stencil() {
-    // a smooth subsonic stencil: every face takes the HLLC star branch
-    for (int s = 0; s < 6; ++s) {
+    // a smooth subsonic stencil: every face takes the HLLC star branch, no
+    // eigenvalue falls in the Steger-Warming or Roe smoothing bands
+    for (int s = 0; s < 8; ++s) {
         const double rho = 1.0 + 0.05 * s, u = 0.3 + 0.01 * s, v = -0.2, w = 0.1,
                      p = 1.0 + 0.02 * s;
         const double q[5] = {rho, rho * u, rho * v, rho * w,
@@ -80,7 +81,7 @@ void set_stencil() {
 void face() {
     Fl cof[3] = {Fl(0.01), Fl(0.0), Fl(0.0)};
     Fl out[5];
-    if (sfo_char_face_flux(g_stencil, cof, g_scheme, g_flux, Fl(1.4), out) != 0) std::abort();
+    if (sfo_char_face_flux_n(g_stencil, cof, g_scheme, g_flux, Fl(1.4), out) != 0) std::abort();
 }
 
 }  // namespace
@@ -89,11 +90,11 @@ int main() {
     set_stencil();
     std::printf("{\n  \"convention\": \"+,-,*,/,sqrt = 1; FMA = 2; pow(x,6) = 3\",\n");
     std::printf("  \"face\": {");
-    const char* sname[2] = {"weno5", "teno5"};
-    const char* fname[2] = {"rusanov", "hllc"};
+    const char* sname[4] = {"weno5", "teno5", "weno3", "weno7"};
+    const char* fname[5] = {"rusanov", "hllc", "steger", "laxfriedrichs", "roe"};
     bool first = true;
-    for (g_scheme = 0; g_scheme < 2; ++g_scheme)
-        for (g_flux = 0; g_flux < 2; ++g_flux) {
+    for (g_scheme = 0; g_scheme < 4; ++g_scheme)
+        for (g_flux = 0; g_flux < 5; ++g_flux) {
             std::printf("%s\"%s_%s\": %llu", first ? "" : ", ", sname[g_scheme], fname[g_flux],
                         counted(face));
             first = false;

@@ -100,10 +100,23 @@ double sfo_weno3_core(const double v[4], unsigned *err)
 }
 
 /* Smoothness of the cubic through four equally spaced values (x = 0..3):
- * the integral over [a, b] of p'^2 + p''^2 + p'''^2, with the cubic in
- * Newton form (forward differences d1, d2/2, d3/6).  Follows beta4,
- * src/methods/numerics/convection/SF_WENO7.cpp:43-68, term by term. */
-static double w7_beta(const double v[4], double a, double b)
+ * the integral over [a, b] of the squared first, second and third
+ * derivatives, with the cubic in Newton form (forward differences d1,
+ * d2/2, d3/6).  Follows beta4,
+ * src/methods/numerics/convection/SF_WENO7.cpp:43-68, term by term; the
+ * interval terms F(b) - F(a) for F = x, x^2, x^3/3, x^4/2, x^5/5 are formed
+ * by the same expressions, once per interval. */
+typedef struct { double i1, i2, i3, i4, i5; } w7_iv;
+
+#define W7_IV(a, b)                                                                    \
+    { (b) - (a), (b) * (b) - (a) * (a), (b) * (b) * (b) / 3.0 - (a) * (a) * (a) / 3.0, \
+      ((b) * (b)) * ((b) * (b)) / 2.0 - ((a) * (a)) * ((a) * (a)) / 2.0,              \
+      (b) * (b) * (b) * (b) * (b) / 5.0 - (a) * (a) * (a) * (a) * (a) / 5.0 }
+
+static const w7_iv W7_INTERVALS[4] = {W7_IV(2.5, 3.5), W7_IV(1.5, 2.5), W7_IV(0.5, 1.5),
+                                      W7_IV(-0.5, 0.5)};
+
+static double w7_beta_iv(const double v[4], const w7_iv *I)
 {
     double d1 = v[1] - v[0];
     double d2 = (v[2] - 2.0 * v[1] + v[0]) * 0.5;
@@ -111,19 +124,19 @@ static double w7_beta(const double v[4], double a, double b)
     double A = d1 - d2 + 2.0 * d3;
     double B = 2.0 * d2 - 6.0 * d3;
     double C = 3.0 * d3;
-    /* F(b) - F(a) for F = x, x^2, x^3/3, x^4/2, x^5/5 */
-    double i1 = b - a;
-    double i2 = b * b - a * a;
-    double i3 = b * b * b / 3.0 - a * a * a / 3.0;
-    double i4 = (b * b) * (b * b) / 2.0 - (a * a) * (a * a) / 2.0;
-    double i5 = b * b * b * b * b / 5.0 - a * a * a * a * a / 5.0;
-    double beta1 = A * A * i1 + A * B * i2 + (B * B + 2.0 * A * C) * i3
-                 + B * C * i4 + C * C * i5;
+    double beta1 = A * A * I->i1 + A * B * I->i2 + (B * B + 2.0 * A * C) * I->i3
+                 + B * C * I->i4 + C * C * I->i5;
     double D = 2.0 * d2 - 6.0 * d3;
     double E = 6.0 * d3;
-    double beta2 = D * D * i1 + D * E * i2 + E * E * i3;
-    double beta3 = 36.0 * d3 * d3 * i1;
+    double beta2 = D * D * I->i1 + D * E * I->i2 + E * E * I->i3;
+    double beta3 = 36.0 * d3 * d3 * I->i1;
     return beta1 + beta2 + beta3;
+}
+
+static double w7_beta(const double v[4], double a, double b)
+{
+    const w7_iv I = W7_IV(a, b);
+    return w7_beta_iv(v, &I);
 }
 
 double sfo_weno7_smoothness(const double v[4], double a, double b)
@@ -146,10 +159,10 @@ double sfo_weno7_core(const double v[7], unsigned *err)
     double p2 = (-v[2] + 7.0 * v[3] + 7.0 * v[4] - v[5]) / 12.0;
     double p3 = (3.0 * v[3] + 13.0 * v[4] - 5.0 * v[5] + v[6]) / 12.0;
     const double d0 = 1.0 / 35.0, d1 = 12.0 / 35.0, d2 = 18.0 / 35.0, d3 = 4.0 / 35.0;
-    double b0 = w7_beta(v + 0, 2.5, 3.5);
-    double b1 = w7_beta(v + 1, 1.5, 2.5);
-    double b2 = w7_beta(v + 2, 0.5, 1.5);
-    double b3 = w7_beta(v + 3, -0.5, 0.5);
+    double b0 = w7_beta_iv(v + 0, &W7_INTERVALS[0]);
+    double b1 = w7_beta_iv(v + 1, &W7_INTERVALS[1]);
+    double b2 = w7_beta_iv(v + 2, &W7_INTERVALS[2]);
+    double b3 = w7_beta_iv(v + 3, &W7_INTERVALS[3]);
     const double eps = 1e-6;
     double a0 = d0 / ((eps + b0) * (eps + b0));
     double a1 = d1 / ((eps + b1) * (eps + b1));

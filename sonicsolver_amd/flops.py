@@ -6,9 +6,16 @@ running the CPU restatement with a counting double type (+, -, *, /, sqrt
 device code performs the same operations (fused where hipcc contracts).
 """
 
-# characteristic reconstruction + face flux, per face (HLLC: subsonic star branch)
-FACE = {("weno5", "rusanov"): 1287, ("weno5", "hllc"): 1328,
-        ("teno5", "rusanov"): 1497, ("teno5", "hllc"): 1538}
+# characteristic reconstruction + face flux, per face (HLLC: subsonic star
+# branch; Steger-Warming and Roe: no eigenvalue in their smoothing bands)
+FACE = {("weno5", "rusanov"): 1287, ("weno5", "hllc"): 1328, ("weno5", "steger"): 1428,
+        ("weno5", "laxfriedrichs"): 1310, ("weno5", "roe"): 1554,
+        ("teno5", "rusanov"): 1497, ("teno5", "hllc"): 1538, ("teno5", "steger"): 1638,
+        ("teno5", "laxfriedrichs"): 1520, ("teno5", "roe"): 1764,
+        ("weno3", "rusanov"): 807, ("weno3", "hllc"): 848, ("weno3", "steger"): 948,
+        ("weno3", "laxfriedrichs"): 830, ("weno3", "roe"): 1074,
+        ("weno7", "rusanov"): 3327, ("weno7", "hllc"): 3368, ("weno7", "steger"): 3468,
+        ("weno7", "laxfriedrichs"): 3350, ("weno7", "roe"): 3594}
 # flux difference and Jacobian scaling, per node, by number of active axes
 RESIDUAL = {1: 20, 2: 30, 3: 40}
 # CFL scan, per node
