@@ -161,3 +161,40 @@ def test_weno7_sod_case_writes_its_outputs(tmp_path):
     m = re.search(r'Name="rho"[^>]*>\n(.*?)\n\s*</DataArray>', text, re.S)
     rho = np.array(m.group(1).split(), dtype=float).reshape(2, 21, 101)
     np.testing.assert_array_equal(rho[1], out[0].reshape(21, 101))
+
+
+SOD_TS = ROOT / "tests" / "cases" / "sod_teno5_steger"
+
+
+def test_reference_sod_case_teno5_steger_warming():
+    """The reference's test/Sod/sodCase tree unchanged (TENO5 + Steger-
+    Warming + SSPRK3 in the pair form, Left without a condition), run to
+    t = 0.2 on the device through the case reader: 20 steps at the 1e-11
+    bar against the oracle, then the whole run in the mean (TENO5 amplifies
+    last-bit differences, as the reference's own builds show)."""
+    with Case(SOD_TS) as c:
+        geom = c.grid()
+        q0 = c.state()
+        conv, time, cfl, max_dt = c.numerics()
+        rc = c.run_control()
+    assert conv == "teno5Steger" and list(geom.bc) == [2, 0, 0]
+    desc = ob.make_desc(n=list(geom.n), active=list(geom.active), bc=list(geom.bc),
+                        d=list(geom.d), gamma=geom.gamma, scheme=int(geom.scheme),
+                        flux=int(geom.flux), time=int(geom.time))
+    print("case sodCase (teno5Steger), 20 steps", flush=True)
+    a, _ = ob.run_driver(desc, q0, 0.0, rc["end_time"], cfl, end_step=20,
+                         write_time=rc["write_interval_time"])
+    with Solver(geom, conv, time, cfl) as s:
+        s.upload(q0)
+        s.run(rc["end_time"], end_step=20, write_interval_time=rc["write_interval_time"])
+        early = s.download()
+    assert_close(early, a, what="sodCase, 20 steps")
+    print("case sodCase (teno5Steger), to t = 0.2", flush=True)
+    ref, dts = ob.run_driver(desc, q0, 0.0, rc["end_time"], cfl,
+                             write_time=rc["write_interval_time"])
+    with Case(SOD_TS) as c:
+        out, steps, t_end = c.run()
+    assert abs(steps - len(dts)) <= 1 and abs(t_end - 0.2) < 1e-15
+    from tests.parity import rel_errors
+    l1, _ = rel_errors(out, ref)
+    assert l1.max() < 1e-4, l1
