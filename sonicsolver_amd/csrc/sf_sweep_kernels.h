@@ -99,8 +99,9 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
                                                              double* __restrict__ S, StageArgs A,
                                                              int seg) {
     constexpr int NS = Stencil<SCHEME>::ns;
-    constexpr int H = NS / 2;   // face a+1/2 reads nodes a+1-H .. a+H
-    __shared__ double win[NS][5][kMarchBlock];
+    constexpr int H = NS / 2;    // face a+1/2 reads nodes a+1-H .. a+H
+    constexpr int W = NS - 2;    // LDS ring slots: nodes a+1-H .. a+H-2
+    __shared__ double win[W][5][kMarchBlock];
     const int lane = threadIdx.x;
     const int i = blockIdx.x * kMarchBlock + lane;
     const int o = blockIdx.y;
@@ -112,30 +113,41 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
         const int j0 = (AXIS == 1) ? a0 : o;
         const int k0 = (AXIS == 2) ? a0 : o;
         const long long step = G.stride[AXIS];
-        // Preload nodes a0-H .. a0+H-2 into slots 1..NS-1; the march starts
+        // The face a+1/2 reads its NS nodes as W from the lane's LDS ring
+        // (nodes a+1-H .. a+H-2), node a+H-1 carried in registers from the
+        // previous face and node a+H loaded for this face.  The march starts
         // one node early (a = a0-1) so that face a0-1/2 is evaluated by the
         // same instructions as every other face: a face's rounding then does
         // not depend on where a segment starts.
-        long long id = node_index(G, i, j0, k0) - H * step;
-        for (int s = 0; s < NS - 1; ++s)
+        long long id = node_index(G, i, j0, k0) - H * step;  // node a0-H
+        for (int s = 0; s < W; ++s)
 #pragma unroll
-            for (int v = 0; v < 5; ++v) win[s + 1][v][lane] = q[v * G.P + id + s * step];
+            for (int v = 0; v < 5; ++v) win[s][v][lane] = q[v * G.P + id + s * step];
+        double carried[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) carried[v] = q[v * G.P + id + W * step];  // node a0+H-2
         id += (H - 1) * step;  // node a0 - 1
-        int head = 0;          // slot that receives the next node
+        int head = 0;          // slot of the oldest node
         double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma clang loop unroll(disable)
         for (int a = a0 - 1; a < a1; ++a, id += step) {
-            // drop the oldest node, append node a+H
-#pragma unroll
-            for (int v = 0; v < 5; ++v) win[head][v][lane] = q[v * G.P + id + H * step];
-            head = (head == NS - 1) ? 0 : head + 1;
             double st[NS][5];
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const int slot = (head + s >= NS) ? head + s - NS : head + s;
+            for (int v = 0; v < 5; ++v) st[NS - 1][v] = q[v * G.P + id + H * step];
+#pragma unroll
+            for (int s = 0; s < W; ++s) {
+                const int slot = (head + s >= W) ? head + s - W : head + s;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) st[s][v] = win[slot][v][lane];
             }
+            // the oldest node leaves the ring, the carried one enters it
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                st[NS - 2][v] = carried[v];
+                win[head][v][lane] = carried[v];
+                carried[v] = st[NS - 1][v];
+            }
+            head = (head == W - 1) ? 0 : head + 1;
             double f[5];
             face_flux<AXIS, SCHEME, FLUX>(st, G.gamma, G.area[AXIS], f, flags);
             double s5[5];
