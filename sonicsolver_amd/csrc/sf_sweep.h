@@ -24,10 +24,11 @@ template <int SCHEME, int FLUX, int MODE>
 void launch_sweep_fm(const SweepCall& c, int axis) {
     const GridDev& G = c.G;
     if (axis == 0) {
-        // up to 16384 blocks, each looping over tiles of 127 faces
+        // up to 16384 blocks (a multiple of 8: the kernel deals tiles to
+        // XCDs), each looping over tiles of kXBlock faces
         const long long faces = (long long)(G.n[0] + 1) * G.n[1] * G.n[2];
         const long long tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
-        const unsigned bx = (unsigned)std::min<long long>(tiles, 16384);  // faces < 2^32 - 2^8
+        const unsigned bx = (unsigned)((std::min<long long>(tiles, 16384) + 7) & ~7LL);
         k_sweep_x<SCHEME, FLUX, MODE><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A);
         return;
     }

@@ -10,14 +10,15 @@ namespace sfamd {
 // ---------------------------------------------------------------------------
 // x sweep over the faces of all x lines numbered as one sequence: face
 // f = line * (n0 + 1) + (a + 1) lies between nodes a and a+1 of its line
-// (a = -1 .. n0-1).  Tile T is faces T*127 .. T*127+127: lane t evaluates
-// face T*127+t and lanes 1..127 form the difference for the node whose upper
-// face is theirs.  Tiles overlap by one face, so every node is updated by
-// exactly one lane, and no tile is left mostly idle at the end of a line.
-// Blocks loop over tiles (stride gridDim.x), so the CFL fold costs one
-// atomic per block.
+// (a = -1 .. n0-1).  A block is one wave; tile T is faces T*63 .. T*63+63:
+// lane t evaluates face T*63+t and lanes 1..63 form the difference for the
+// node whose upper face is theirs, taking the lower face's flux from lane
+// t-1 by a cross-lane shift (no LDS, no barrier).  Tiles overlap by one
+// face, so every node is updated by exactly one lane, and no tile is left
+// mostly idle at the end of a line.  Blocks loop over tiles (stride
+// gridDim.x), so the CFL fold costs one atomic per block.
 // ---------------------------------------------------------------------------
-constexpr int kXBlock = 128;
+constexpr int kXBlock = 64;
 
 // The final HLLC sweeps fit four waves per SIMD (128 VGPRs) without
 // spilling when asked to; the others stay at three (the final Rusanov sweeps
@@ -26,8 +27,6 @@ template <int SCHEME, int FLUX, int MODE>
 __global__ __launch_bounds__(kXBlock)
 __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) ? 4 : 3, 8))) void k_sweep_x(GridDev G, const double* __restrict__ q,
                                                      double* __restrict__ S, StageArgs A) {
-    __shared__ double sh[5][kXBlock];
-    __shared__ unsigned long long red[kXBlock / 64];
     // face numbers fit 32 bits: the host rejects grids with 2^32 - 2^8
     // faces or more along x
     const int t = threadIdx.x;
@@ -36,12 +35,19 @@ __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) 
     const unsigned tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
     unsigned flags = 0;
     double dt_local = INFINITY;
-    for (unsigned T = blockIdx.x; T < tiles; T += gridDim.x) {
+    // Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
+    // each with its own L2; tiles are numbered so that the blocks of one XCD
+    // take consecutive tiles and share the stencil halo between them in that
+    // L2.  The grid size is a multiple of 8.
+    const unsigned per_xcd = gridDim.x >> 3;
+    const unsigned first = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    for (unsigned T = first; T < tiles; T += gridDim.x) {
         const unsigned f = T * (kXBlock - 1) + t;
         const bool face = f < faces;
         const unsigned line = face ? f / per_line : 0;
         const int a = face ? (int)(f - line * per_line) - 1 : 0;  // lower node of the face
         const int j = (int)(line % (unsigned)G.n[1]), k = (int)(line / (unsigned)G.n[1]);
+        double f5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (face) {
             constexpr int NS = Stencil<SCHEME>::ns;
             const long long base = node_index(G, a, j, k);
@@ -50,21 +56,20 @@ __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) 
             for (int s = 0; s < NS; ++s)
 #pragma unroll
                 for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + base + (s + 1 - NS / 2)];
-            double f5[5];
             face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
+            for (int v = 0; v < 5; ++v)
                 if (!isfinite(f5[v])) flags |= kFlagFlux;
-                sh[v][t] = f5[v];
-            }
         }
-        __syncthreads();
+        double lower[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) lower[v] = __shfl_up(f5[v], 1, kXBlock);
         if (t >= 1 && face && a >= 0) {
             const long long id = node_index(G, a, j, k);
             double s5[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
-                const double d = sh[v][t] - sh[v][t - 1];
+                const double d = f5[v] - lower[v];
                 s5[v] = (MODE == kCopyS) ? d : S[v * G.P + id] + d;
             }
             if (MODE == kFinal) {
@@ -77,10 +82,9 @@ __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) 
                 for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
             }
         }
-        __syncthreads();  // sh is rewritten by the next tile
     }
     raise_flags(flags, A.flags);
-    if (MODE == kFinal && A.reduce_dt) block_min_atomic(dt_local, A.dtmin, red);
+    if (MODE == kFinal && A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);
 }
 
 // ---------------------------------------------------------------------------
