@@ -219,8 +219,11 @@ int sf_batch_add(sf_batch *batch, const sf_batch_item *item, const double *soa,
                  long long max_steps, int *slot);
 int sf_batch_remove(sf_batch *batch, int slot);
 int sf_batch_download(sf_batch *batch, int slot, double *soa);
-/* Advance every live problem by nsteps steps; use_graph replays one
- * captured hipGraph per step (captured again after admissions/removals). */
+/* Advance every live problem by nsteps steps.  use_graph 0 launches each
+ * step's kernels directly; 1 replays one captured hipGraph per step
+ * (captured again after admissions/removals); 2 replays the graph when one
+ * is current or nsteps is long enough to pay for capturing it, and
+ * launches directly otherwise.  Results do not depend on it. */
 int sf_batch_step(sf_batch *batch, int nsteps, int use_graph);
 /* Physical time and step count of one problem. */
 int sf_batch_clock(sf_batch *batch, int slot, double *time, long long *steps);

@@ -15,6 +15,7 @@
 // q[v*P + off_p + (j+gy)*m0 + i+3] with m0 = nx + 6, gy = 3 for a 2D
 // problem and 0 for a 1D one.
 #include "sonic_amd.h"
+#include "sf_batch_copy.h"
 #include "sf_device.h"
 
 #include <hip/hip_runtime.h>
@@ -216,8 +217,8 @@ __attribute__((amdgpu_waves_per_eu((SCHEME == 1 && FLUX == 1) ? 2 : 3, 8))) void
 // problem's y work is numbered (segment, column) = k -> (k / nx, k % nx),
 // segments of yseg nodes; a tile is 64 consecutive k, one per lane, so
 // narrow problems fill whole waves.  Each lane marches up its column
-// segment with the 6-node stencil window in LDS (one load per node) and
-// writes S = F(j+1/2) - F(j-1/2).
+// segment with the stencil window in LDS and registers (one load per node)
+// and writes S = F(j+1/2) - F(j-1/2).
 template <int SCHEME, int FLUX>
 __device__ __forceinline__ void y_tiles(const BatchDev& B, const double* __restrict__ q,
                                         double* __restrict__ S, int t0, int t1, int b0, int nb,
@@ -236,28 +237,38 @@ __device__ __forceinline__ void y_tiles(const BatchDev& B, const double* __restr
         const int a0 = seg * B.yseg;
         const int a1 = min(a0 + B.yseg, it.n1);
         const long long step = it.m0;
-        // nodes a0-3 .. a0+1 into slots 1..5; the march starts at a0-1 so
-        // every face is evaluated by the same instructions (see
-        // k_sweep_march)
-        long long id = item_index(it, i, a0) - 3 * step;
-        for (int s = 0; s < 5; ++s)
+        // the face a+1/2 reads nodes a-2 .. a+1 from the lane's 4-slot LDS
+        // ring, node a+2 carried in registers, node a+3 loaded for it; the
+        // march starts at a0-1 so every face is evaluated by the same
+        // instructions (see k_sweep_march)
+        long long id = item_index(it, i, a0) - 3 * step;  // node a0-3
+        for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int v = 0; v < 5; ++v) win[s + 1][v][lane] = q[v * B.P + id + s * step];
+            for (int v = 0; v < 5; ++v) win[s][v][lane] = q[v * B.P + id + s * step];
+        double carried[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) carried[v] = q[v * B.P + id + 4 * step];
         id += 2 * step;
         int head = 0;
         double fprev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma clang loop unroll(disable)
         for (int a = a0 - 1; a < a1; ++a, id += step) {
-#pragma unroll
-            for (int v = 0; v < 5; ++v) win[head][v][lane] = q[v * B.P + id + 3 * step];
-            head = (head == 5) ? 0 : head + 1;
             double stc[6][5];
 #pragma unroll
-            for (int s = 0; s < 6; ++s) {
-                const int slot = (head + s >= 6) ? head + s - 6 : head + s;
+            for (int v = 0; v < 5; ++v) stc[5][v] = q[v * B.P + id + 3 * step];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int slot = (head + s) & 3;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) stc[s][v] = win[slot][v][lane];
             }
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                stc[4][v] = carried[v];
+                win[head][v][lane] = carried[v];
+                carried[v] = stc[5][v];
+            }
+            head = (head + 1) & 3;
             double f[5];
             face_flux<1, SCHEME, FLUX>(stc, B.gamma, it.area1, f, flags);
             double d[5];
@@ -280,7 +291,7 @@ __device__ __forceinline__ void y_tiles(const BatchDev& B, const double* __restr
 
 __global__ __launch_bounds__(64) void k_batch_y_fused(BatchDev B, const double* __restrict__ q,
                                                       double* __restrict__ S, Split sp) {
-    __shared__ double win[6][5][64];
+    __shared__ double win[4][5][64];
     const int bid = blockIdx.x;
     int v = 0;
     while (v < 3 && bid >= sp.block[v + 1]) ++v;
@@ -293,10 +304,12 @@ __global__ __launch_bounds__(64) void k_batch_y_fused(BatchDev B, const double* 
     }
 }
 
+// The WENO5 variants fit four waves per SIMD (128 VGPRs) when asked to.
 template <int SCHEME, int FLUX>
-__global__ __launch_bounds__(64) void k_batch_y(BatchDev B, const double* __restrict__ q,
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu((SCHEME == 0) ? 4 : 3, 8))) void k_batch_y(BatchDev B, const double* __restrict__ q,
                                                 double* __restrict__ S, int t0, int t1) {
-    __shared__ double win[6][5][64];
+    __shared__ double win[4][5][64];
     y_tiles<SCHEME, FLUX>(B, q, S, t0, t1, blockIdx.x, gridDim.x, win);
 }
 
@@ -448,6 +461,8 @@ struct sf_batch {
     long long capacity = 0;     // padded nodes per variable
     int item_capacity = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;   // copy-out of finished problems
+    hipEvent_t copy_after = nullptr;     // orders copy-outs after the compute stream
     double* buf[3] = {nullptr, nullptr, nullptr};
     double* S = nullptr;
     double* staging = nullptr;
@@ -733,7 +748,13 @@ int prepare_pending(sf_batch* b) {
 }
 
 // Queue nsteps steps (graph replay or direct launches); no wait.
+// Steps from which capturing a new graph pays for itself against direct
+// launches (a capture and instantiation cost about as much as a dozen
+// steps' worth of kernel launches).
+constexpr int kGraphCaptureSteps = 16;
+
 int advance(sf_batch* b, int nsteps, int use_graph) {
+    if (use_graph == 2) use_graph = (b->graph_exec || nsteps >= kGraphCaptureSteps) ? 1 : 0;
     if (use_graph) {
         if (!b->graph_exec) {
             SF_BHIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -808,6 +829,8 @@ int sf_batch_create(int device, double gamma, size_t node_capacity, int item_cap
     ok = ok && hipMalloc(&b->d_active, sizeof(int) * item_capacity) == hipSuccess;
     ok = ok && hipMalloc(&b->d_scan, sizeof(int) * item_capacity) == hipSuccess;
     ok = ok && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&b->copy_stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&b->copy_after, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipMemset(b->d_flags, 0, sizeof(unsigned) * item_capacity) == hipSuccess;
     ok = ok && hipMemset(b->d_steps, 0, sizeof(long long) * item_capacity) == hipSuccess;
     ok = ok && hipMemset(b->d_target, 0, sizeof(long long) * item_capacity) == hipSuccess;
@@ -831,6 +854,7 @@ int sf_batch_destroy(sf_batch* b) {
     if (!b) return 0;
     hipSetDevice(b->device);
     if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->copy_stream) hipStreamSynchronize(b->copy_stream);
     drop_graph(b);
     for (double* p : b->buf)
         if (p) hipFree(p);
@@ -841,9 +865,12 @@ int sf_batch_destroy(sf_batch* b) {
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);
+    if (b->copy_stream) hipStreamDestroy(b->copy_stream);
+    if (b->copy_after) hipEventDestroy(b->copy_after);
     delete b;
     return 0;
 }
+
 
 int sf_batch_add(sf_batch* b, const sf_batch_item* item, const double* soa, long long max_steps,
                  int* slot) {
@@ -961,7 +988,8 @@ int sf_batch_download(sf_batch* b, int slot, double* soa) {
 }
 
 int sf_batch_step(sf_batch* b, int nsteps, int use_graph) {
-    if (!b || nsteps < 0) return bfail(SF_E_ARG, "sf_batch_step: invalid argument");
+    if (!b || nsteps < 0 || use_graph < 0 || use_graph > 2)
+        return bfail(SF_E_ARG, "sf_batch_step: invalid argument");
     SF_BHIP(hipSetDevice(b->device));
     if (b->tables_dirty)
         if (int rc = rebuild_tables(b)) return rc;
@@ -1190,3 +1218,42 @@ extern "C" int sf_batch_set_occupancy(sf_batch* b, int blocks_per_cu) {
     drop_graph(b);
     return 0;
 }
+
+namespace sfamd {
+
+long long batch_copy_doubles(const sf_batch* b, int slot) {
+    const BatchItemDev& it = b->items[(size_t)slot];
+    return 5LL * it.m0 * (it.n1 + 2 * it.gy) + 1;
+}
+
+int batch_copy_out_async(sf_batch* b, int slot, double* dst, hipEvent_t done) {
+    if (!valid_slot(b, slot)) return bfail(SF_E_ARG, "batch copy-out: invalid slot");
+    SF_BHIP(hipSetDevice(b->device));
+    const BatchItemDev& it = b->items[(size_t)slot];
+    const long long extent = (long long)it.m0 * (it.n1 + 2 * it.gy);
+    SF_BHIP(hipEventRecord(b->copy_after, b->stream));
+    SF_BHIP(hipStreamWaitEvent(b->copy_stream, b->copy_after, 0));
+    // the state after a whole step is in buf[0]
+    for (int v = 0; v < 5; ++v)
+        SF_BHIP(hipMemcpyAsync(dst + v * extent, b->buf[0] + v * b->capacity + it.off,
+                               sizeof(double) * (size_t)extent, hipMemcpyDeviceToHost,
+                               b->copy_stream));
+    SF_BHIP(hipMemcpyAsync(dst + 5 * extent, b->d_time + slot, sizeof(double),
+                           hipMemcpyDeviceToHost, b->copy_stream));
+    SF_BHIP(hipEventRecord(done, b->copy_stream));
+    return 0;
+}
+
+void batch_unpack(const sf_batch* b, int slot, const double* src, double* soa, double* time) {
+    const BatchItemDev& it = b->items[(size_t)slot];
+    const long long extent = (long long)it.m0 * (it.n1 + 2 * it.gy);
+    const long long N = (long long)it.n0 * it.n1;
+    for (int v = 0; v < 5; ++v)
+        for (int j = 0; j < it.n1; ++j) {
+            const double* row = src + v * extent + (long long)(j + it.gy) * it.m0 + 3;
+            std::copy(row, row + it.n0, soa + v * N + (long long)j * it.n0);
+        }
+    *time = src[5 * extent];
+}
+
+}  // namespace sfamd

@@ -9,8 +9,14 @@
 // (capped at steps_per_tick), so a problem finishes exactly at a pass
 // boundary, and (4) hands finished problems back and frees their slots.
 // Each problem stops at its own step count on the device (sf_batch_add's
-// max_steps), so it never takes a step too many.
+// max_steps), so it never takes a step too many.  A finished problem's
+// state is copied out on the batch's copy stream into pinned host memory
+// while the next pass computes; its slot is freed and its result handed
+// back once that copy has completed.
 #include "sonic_amd_serve.h"
+#include "sf_batch_copy.h"
+
+#include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
@@ -95,6 +101,19 @@ struct sf_server {
     // loop thread only
     std::deque<RequestPtr> waiting;
     std::vector<RequestPtr> running;
+    struct Pinned {
+        double* p = nullptr;
+        long long cap = 0;      // doubles
+        bool busy = false;
+    };
+    struct Outgoing {
+        RequestPtr r;
+        int buffer = -1;        // index into pinned
+        hipEvent_t done = nullptr;
+    };
+    std::vector<Pinned> pinned;
+    std::vector<hipEvent_t> events;  // free copy-out events
+    std::vector<Outgoing> outgoing;  // copy-outs in flight
     std::thread worker;
 };
 
@@ -153,6 +172,71 @@ void admit(sf_server* s) {
     }
 }
 
+// A pinned host buffer of at least n doubles (reused; grown on demand).
+int pinned_buffer(sf_server* s, long long n) {
+    int best = -1;
+    for (int i = 0; i < (int)s->pinned.size(); ++i)
+        if (!s->pinned[i].busy && s->pinned[i].cap >= n &&
+            (best < 0 || s->pinned[i].cap < s->pinned[best].cap))
+            best = i;
+    if (best >= 0) return best;
+    sf_server::Pinned b;
+    if (hipHostMalloc((void**)&b.p, sizeof(double) * (size_t)n, hipHostMallocDefault) != hipSuccess)
+        return -1;
+    b.cap = n;
+    s->pinned.push_back(b);
+    return (int)s->pinned.size() - 1;
+}
+
+// Start the copy-out of a finished problem; on failure the request fails.
+void start_copy_out(sf_server* s, const RequestPtr& r) {
+    hipEvent_t ev = nullptr;
+    if (!s->events.empty()) {
+        ev = s->events.back();
+        s->events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        sf_batch_remove(s->batch, r->slot);
+        finish(s, r, -1, "copy-out: event creation failed");
+        return;
+    }
+    const int buf = pinned_buffer(s, sfamd::batch_copy_doubles(s->batch, r->slot));
+    if (buf < 0 || sfamd::batch_copy_out_async(s->batch, r->slot, s->pinned[buf].p, ev) != 0) {
+        const std::string why = buf < 0 ? "pinned host allocation failed" : last_error();
+        s->events.push_back(ev);
+        sf_batch_remove(s->batch, r->slot);
+        finish(s, r, -1, "copy-out: " + why);
+        return;
+    }
+    s->pinned[buf].busy = true;
+    s->outgoing.push_back({r, buf, ev});
+}
+
+// Hand back the problems whose copy-out has completed (all of them when
+// `wait`), freeing their slots.
+void collect(sf_server* s, bool wait) {
+    for (auto it = s->outgoing.begin(); it != s->outgoing.end();) {
+        const hipError_t e = wait ? hipEventSynchronize(it->done) : hipEventQuery(it->done);
+        if (e == hipErrorNotReady) {
+            ++it;
+            continue;
+        }
+        const RequestPtr r = it->r;
+        sf_server::Pinned& b = s->pinned[(size_t)it->buffer];
+        std::string err;
+        if (e == hipSuccess) {
+            r->state.resize((size_t)5 * (size_t)r->nodes);
+            sfamd::batch_unpack(s->batch, r->slot, b.p, r->state.data(), &r->time);
+        } else {
+            err = std::string("copy-out: ") + hipGetErrorString(e);
+        }
+        b.busy = false;
+        s->events.push_back(it->done);
+        sf_batch_remove(s->batch, r->slot);
+        it = s->outgoing.erase(it);
+        finish(s, r, err.empty() ? 1 : -1, err);
+    }
+}
+
 void drop_running(sf_server* s, const std::string& why) {
     for (const RequestPtr& r : s->running) {
         sf_batch_remove(s->batch, r->slot);
@@ -164,11 +248,24 @@ void drop_running(sf_server* s, const std::string& why) {
 void loop(sf_server* s) {
     std::vector<long long> steps((size_t)s->max_items);
     for (;;) {
+        collect(s, false);
+        if (s->waiting.empty() && s->running.empty() && !s->outgoing.empty()) {
+            bool idle;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                idle = s->incoming.empty() && !s->stop;
+            }
+            if (idle) collect(s, true);
+        }
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            if (s->waiting.empty() && s->running.empty())
+            if (s->waiting.empty() && s->running.empty() && s->outgoing.empty())
                 s->cv_work.wait(lk, [&] { return s->stop || !s->incoming.empty(); });
-            if (s->stop) return;
+            if (s->stop) {
+                lk.unlock();
+                collect(s, true);
+                return;
+            }
             while (!s->incoming.empty()) {
                 s->waiting.push_back(std::move(s->incoming.front()));
                 s->incoming.pop_front();
@@ -188,8 +285,13 @@ void loop(sf_server* s) {
             s->admit_s += d;
         }
         if (s->running.empty()) {
-            // nothing fits and nothing runs: only possible after failures;
-            // wait for new work rather than spin
+            // nothing runs: either copy-outs still hold the memory the
+            // waiting requests need, or nothing fits at all (only possible
+            // after failures); wait for the copies or fail the waiting
+            if (!s->outgoing.empty()) {
+                collect(s, true);
+                continue;
+            }
             if (!s->waiting.empty()) {
                 for (const RequestPtr& r : s->waiting) finish(s, r, -1, "cannot be admitted");
                 s->waiting.clear();
@@ -200,7 +302,7 @@ void loop(sf_server* s) {
         for (const RequestPtr& r : s->running) k = std::min(k, r->steps - r->steps_done);
         k = std::max(1LL, k);
         const auto t0 = Clock::now();
-        int rc = sf_batch_step(s->batch, (int)k, 1);
+        int rc = sf_batch_step(s->batch, (int)k, 2);
         const double dt = std::chrono::duration<double>(Clock::now() - t0).count();
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -244,14 +346,8 @@ void loop(sf_server* s) {
                 ++it;
                 continue;
             }
-            r->state.resize((size_t)5 * (size_t)r->nodes);
-            long long ns = 0;
-            int rc2 = sf_batch_download(s->batch, r->slot, r->state.data());
-            if (rc2 == 0) rc2 = sf_batch_clock(s->batch, r->slot, &r->time, &ns);
-            const std::string err = rc2 == 0 ? std::string() : last_error();
-            sf_batch_remove(s->batch, r->slot);
             it = s->running.erase(it);
-            finish(s, r, rc2 == 0 ? 1 : -1, err);
+            start_copy_out(s, r);
         }
         const double d = std::chrono::duration<double>(Clock::now() - tr).count();
         std::lock_guard<std::mutex> lk(s->mu);
@@ -311,6 +407,9 @@ int sf_server_destroy(sf_server* s) {
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
     sf_batch_destroy(s->batch);
+    for (auto& b : s->pinned)
+        if (b.p) hipHostFree(b.p);
+    for (hipEvent_t e : s->events) hipEventDestroy(e);
     delete s;
     return 0;
 }
