@@ -211,7 +211,8 @@ def configs_1d_2d():
 
 def recipes_run(n: int, steps: int):
     """Every registered convection recipe (WENO3 / WENO5 / TENO5 / WENO7 x
-    Steger-Warming / Rusanov / Lax-Friedrichs / Roe, plus HLLC) on the same
+    Steger-Warming / Rusanov / Lax-Friedrichs / Roe / Lax-Wendroff, plus
+    HLLC) on the same
     n^3 periodic Taylor-Green-type field with SSPRK3 at CFL 0.4: steps timed
     after one warm-up step, the state resident on the device.  Reported as
     cell updates per second and as FP64 rate against the per-recipe
@@ -220,7 +221,7 @@ def recipes_run(n: int, steps: int):
     kw, q = wl.tgv_3d(n)
     out = {}
     for scheme in ("weno3", "weno5", "teno5", "weno7"):
-        for flux in ("steger", "rusanov", "laxfriedrichs", "roe", "hllc"):
+        for flux in ("steger", "rusanov", "laxfriedrichs", "roe", "laxwendroff", "hllc"):
             kw2 = dict(kw, scheme=scheme, flux=flux)
             with Grid(make_desc(**kw2)) as g:
                 g.upload(q)

@@ -34,7 +34,7 @@ extern "C" {
 enum { SF_SCHEME_WENO5 = 0, SF_SCHEME_TENO5 = 1, SF_SCHEME_WENO3 = 2, SF_SCHEME_WENO7 = 3 };
 /* FDM::FluxSplitter, SF_termRecipe.h:34-41; HLLC has no reference member */
 enum { SF_FLUX_RUSANOV = 0, SF_FLUX_HLLC = 1, SF_FLUX_STEGER_WARMING = 2,
-       SF_FLUX_LAX_FRIEDRICHS = 3, SF_FLUX_ROE = 4 };
+       SF_FLUX_LAX_FRIEDRICHS = 3, SF_FLUX_ROE = 4, SF_FLUX_LAX_WENDROFF = 5 };
 /* FDM::TimeRecipeId, core/config/types/SF_timeRecipe.h:10 */
 enum { SF_TIME_EULER = 0, SF_TIME_SSPRK3 = 1, SF_TIME_RK4 = 2 };
 /* Boundary kind of an axis: zeroGradient (transmissive) sets on both

@@ -24,7 +24,8 @@ _P = ctypes.POINTER(ctypes.c_double)
 _U = ctypes.POINTER(ctypes.c_uint)
 
 SCHEMES = {"weno5": 0, "teno5": 1, "weno3": 2, "weno7": 3}
-FLUXES = {"rusanov": 0, "hllc": 1, "steger": 2, "laxfriedrichs": 3, "roe": 4}
+FLUXES = {"rusanov": 0, "hllc": 1, "steger": 2, "laxfriedrichs": 3, "roe": 4,
+          "laxwendroff": 5}
 TIMES = {"euler": 0, "forwardeuler": 0, "ssprk3": 1, "rk4": 2, "classicalrk4": 2}
 # per axis; "none_lo" / "none_hi" / "none": that end carries no condition
 BCS = {"transmissive": 0, "periodic": 1, "none_lo": 2, "none_hi": 3, "none": 4}
@@ -97,6 +98,11 @@ def oracle():
         lib.sfo_face_flux.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.c_double, _P]
         lib.sfo_char_face_flux_n.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_double, _P]
+        lib.sfo_char_face_flux_dt.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                              _P]
+        lib.sfo_lax_wendroff.argtypes = [_P, _P, _P, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, _P]
         lib.sfo_char_matrix.argtypes = [_P, _P, ctypes.c_double, _P, _P]
         for name in ("sfo_rusanov", "sfo_hllc"):
             getattr(lib, name).argtypes = [_P, _P, _P, ctypes.c_double, _P]
@@ -154,6 +160,10 @@ def _declare_ref(lib):
         getattr(lib, name).argtypes = [_P]
         getattr(lib, name).restype = ctypes.c_double
     lib.sfref_split_flux.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.c_double, _P]
+    lib.sfref_lax_wendroff.argtypes = [_P, _P, _P, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, _P]
+    lib.sfref_char_face_flux_dt.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, _P]
     lib.sfref_run_driver.argtypes = [ctypes.POINTER(Desc), _P, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, _P,
@@ -342,11 +352,20 @@ def physical_flux(q, n, gamma=1.4):
     return rc, out
 
 
-def char_face_flux(stencil, cofactor, scheme="weno5", flux="rusanov", gamma=1.4):
-    """stencil: (ns, 5) nodes at offsets 1-ns/2 .. ns/2 of the face."""
+def char_face_flux(stencil, cofactor, scheme="weno5", flux="rusanov", gamma=1.4, dt=0.0,
+                   spacing=0.0):
+    """stencil: (ns, 5) nodes at offsets 1-ns/2 .. ns/2 of the face; dt and
+    spacing are read by the Lax-Wendroff flux only."""
     out = np.zeros(5)
     sid = SCHEMES[scheme]
     st = _arr(stencil).reshape(STENCIL[sid] * 5)
-    rc = oracle().sfo_char_face_flux_n(_ptr(st), _ptr(_arr(cofactor)), sid, FLUXES[flux],
-                                       gamma, _ptr(out))
+    rc = oracle().sfo_char_face_flux_dt(_ptr(st), _ptr(_arr(cofactor)), sid, FLUXES[flux],
+                                        gamma, dt, spacing, _ptr(out))
+    return rc, out
+
+
+def lax_wendroff(qL, qR, n, dt, spacing, gamma=1.4):
+    out = np.zeros(5)
+    rc = oracle().sfo_lax_wendroff(_ptr(_arr(qL)), _ptr(_arr(qR)), _ptr(_arr(n)), gamma, dt,
+                                   spacing, _ptr(out))
     return rc, out

@@ -81,7 +81,8 @@ void set_stencil() {
 void face() {
     Fl cof[3] = {Fl(0.01), Fl(0.0), Fl(0.0)};
     Fl out[5];
-    if (sfo_char_face_flux_n(g_stencil, cof, g_scheme, g_flux, Fl(1.4), out) != 0) std::abort();
+    if (sfo_char_face_flux_dt(g_stencil, cof, g_scheme, g_flux, Fl(1.4), Fl(0.01), Fl(0.1), out) != 0)
+        std::abort();
 }
 
 }  // namespace
@@ -91,10 +92,10 @@ int main() {
     std::printf("{\n  \"convention\": \"+,-,*,/,sqrt = 1; FMA = 2; pow(x,6) = 3\",\n");
     std::printf("  \"face\": {");
     const char* sname[4] = {"weno5", "teno5", "weno3", "weno7"};
-    const char* fname[5] = {"rusanov", "hllc", "steger", "laxfriedrichs", "roe"};
+    const char* fname[6] = {"rusanov", "hllc", "steger", "laxfriedrichs", "roe", "laxwendroff"};
     bool first = true;
     for (g_scheme = 0; g_scheme < 4; ++g_scheme)
-        for (g_flux = 0; g_flux < 5; ++g_flux) {
+        for (g_flux = 0; g_flux < 6; ++g_flux) {
             std::printf("%s\"%s_%s\": %llu", first ? "" : ", ", sname[g_scheme], fname[g_flux],
                         counted(face));
             first = false;

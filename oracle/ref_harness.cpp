@@ -54,6 +54,7 @@ using Model = SF::Physics::FluidStateModel::Model;
 
 struct Case {
     sfo_desc d;
+    double dt = 0.0;   // the step's dt (state.dt), read by the Lax-Wendroff flux
     std::shared_ptr<const Model> eq;
     SF::Field field;
     SF::Boundary::Applicator bc;
@@ -232,6 +233,7 @@ SF::FDM::FluxSplitter splitter_of(int flux) {
     case SFO_STEGER: return SF::FDM::FluxSplitter::StegerWarming;
     case SFO_LAXF: return SF::FDM::FluxSplitter::LaxFriedrichs;
     case SFO_ROE: return SF::FDM::FluxSplitter::Roe;
+    case SFO_LAXW: return SF::FDM::FluxSplitter::LaxWendroff;
     }
     return SF::FDM::FluxSplitter::Rusanov;
 }
@@ -263,7 +265,7 @@ void assemble(Case& c) {
     const double gamma = c.d.gamma;
     if (c.d.flux != SFO_HLLC) {
         SF::divDispatch(c.field, w.convectiveFlux, w.residual, scheme_of(c.d.scheme),
-                        splitter_of(c.d.flux), 0.0,
+                        splitter_of(c.d.flux), c.dt,
                         SF::FDM::IBMBoundaryScheme::LowOrder, 0, gamma, *c.eq);
         return;
     }
@@ -341,6 +343,13 @@ int sfref_split_flux(int flux, const double qL[5], const double qR[5], const dou
     });
 }
 
+int sfref_lax_wendroff(const double qL[5], const double qR[5], const double n[3], double gamma,
+                       double dt, double spacing, double out[5]) {
+    return guarded([&] {
+        SF::Flux::LaxWendroffFlux::flux(qL, qR, n, dt, spacing, out, gamma);
+    });
+}
+
 int sfref_char_matrix(const double q[5], const double n[3], double gamma,
                       double L[25], double R[25]) {
     return guarded([&] { SF::Riemann::buildCharacteristicMatrix(q, n, gamma, L, R); });
@@ -360,7 +369,7 @@ int sfref_rusanov(const double qL[5], const double qR[5], const double n[3],
 namespace {
 template <int NS, class Core>
 void char_face(const double* st, const double cof[3], Core core, int flux, double gamma,
-               double out[5]) {
+               double dt, double spacing, double out[5]) {
     double stencil[NS][5];
     for (int s = 0; s < NS; ++s)
         for (int v = 0; v < 5; ++v) stencil[s][v] = st[5 * s + v];
@@ -372,6 +381,7 @@ void char_face(const double* st, const double cof[3], Core core, int flux, doubl
         case SFO_STEGER: SF::Flux::StegerWarmingFlux::flux(qL, qR, n, o, gamma); return;
         case SFO_LAXF: SF::Flux::LaxFriedrichsFlux::flux(qL, qR, n, o, gamma); return;
         case SFO_ROE: SF::Flux::RoeFlux::flux(qL, qR, n, o, gamma); return;
+        case SFO_LAXW: SF::Flux::LaxWendroffFlux::flux(qL, qR, n, dt, spacing, o, gamma); return;
         }
         SF::Numerics::RusanovEOS::faceFlux(*eq, qL, qR, {n[0], n[1], n[2]}, o, 5);
     };
@@ -384,16 +394,21 @@ extern "C" {
 
 // characteristicWENOFaceFlux<NS> for any scheme / flux id; the stencil
 // holds sfo_stencil_size(scheme) nodes.
-int sfref_char_face_flux_n(const double* st, const double cof[3], int scheme, int flux,
-                           double gamma, double out[5]) {
+int sfref_char_face_flux_dt(const double* st, const double cof[3], int scheme, int flux,
+                            double gamma, double dt, double spacing, double out[5]) {
     return guarded([&] {
         switch (scheme) {
-        case SFO_TENO5: char_face<6>(st, cof, SF::TENO5::teno5_core, flux, gamma, out); break;
-        case SFO_WENO3: char_face<4>(st, cof, SF::WENO3::weno3_core, flux, gamma, out); break;
-        case SFO_WENO7: char_face<8>(st, cof, SF::WENO7::weno7_core, flux, gamma, out); break;
-        default: char_face<6>(st, cof, SF::WENO5::weno5_core, flux, gamma, out);
+        case SFO_TENO5: char_face<6>(st, cof, SF::TENO5::teno5_core, flux, gamma, dt, spacing, out); break;
+        case SFO_WENO3: char_face<4>(st, cof, SF::WENO3::weno3_core, flux, gamma, dt, spacing, out); break;
+        case SFO_WENO7: char_face<8>(st, cof, SF::WENO7::weno7_core, flux, gamma, dt, spacing, out); break;
+        default: char_face<6>(st, cof, SF::WENO5::weno5_core, flux, gamma, dt, spacing, out);
         }
     });
+}
+
+int sfref_char_face_flux_n(const double* st, const double cof[3], int scheme, int flux,
+                           double gamma, double out[5]) {
+    return sfref_char_face_flux_dt(st, cof, scheme, flux, gamma, 0.0, 0.0, out);
 }
 
 int sfref_char_face_flux(const double st[30], const double cof[3], int scheme,
@@ -433,6 +448,7 @@ double step_case(Case& c, SF::State::StateBundle& state,
     double dt = SF::Numerics::RusanovEOS::deltaT(c.field, *c.eq, cfl);
     dt = std::min({dt, max_dt, limit});
     state.dt = dt;
+    c.dt = dt;
     SF::Time::Explicit::Workspace workspace;
     SF::Time::Explicit::begin(workspace, state.patches, state, recipe, nullptr);
     for (int st = 0; st < recipe.stageCount(); ++st) {
@@ -510,6 +526,7 @@ int sfref_run(const sfo_desc* d, double* soa_io, int nsteps, double cfl,
         double dt = SF::Numerics::RusanovEOS::deltaT(c->field, *c->eq, cfl);
         dt = std::min(dt, max_dt);
         state.dt = dt;
+        c->dt = dt;
         if (dt_seq) dt_seq[s] = dt;
         SF::Time::Explicit::Workspace workspace;
         SF::Time::Explicit::begin(workspace, state.patches, state, recipe, nullptr);

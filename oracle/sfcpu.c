@@ -592,6 +592,23 @@ int sfo_roe(const double qL[5], const double qR[5], const double n[3],
     return 0;
 }
 
+/* LaxWendroffFlux::flux, src/methods/numerics/Flux/SF_lw.cpp:17-40: the
+ * physical flux of the Richtmyer half state
+ * q* = (qL + qR)/2 - (dt/spacing)/2 (F(qR) - F(qL)); dt is the step's
+ * (AssemblyContext::timeStep = state.dt, SF_conservativeRHS.cpp:167-173). */
+int sfo_lax_wendroff(const double qL[5], const double qR[5], const double n[3],
+                     double gamma, double dt, double spacing, double out[5])
+{
+    if (!(dt > 0.0) || !(spacing > 0.0)) return -1;
+    double fL[5], fR[5];
+    if (euler_flux(qL, n, gamma, fL) || euler_flux(qR, n, gamma, fR)) return -1;
+    const double lambda = dt / spacing;
+    double qh[5];
+    for (int v = 0; v < 5; ++v)
+        qh[v] = 0.5 * (qL[v] + qR[v]) - 0.5 * lambda * (fR[v] - fL[v]);
+    return euler_flux(qh, n, gamma, out);
+}
+
 int sfo_face_flux(int flux, const double qL[5], const double qR[5], const double n[3],
                   double gamma, double out[5])
 {
@@ -611,6 +628,13 @@ int sfo_face_flux(int flux, const double qL[5], const double qR[5], const double
  * src/solver/discretization/convection/SF_WENO.h:64-98,185-245. */
 int sfo_char_face_flux_n(const double *st, const double cof[3], int scheme,
                          int flux, double gamma, double out[5])
+{
+    return sfo_char_face_flux_dt(st, cof, scheme, flux, gamma, 0.0, 0.0, out);
+}
+
+int sfo_char_face_flux_dt(const double *st, const double cof[3], int scheme,
+                          int flux, double gamma, double dt, double spacing,
+                          double out[5])
 {
     const int ns = sfo_stencil_size(scheme);
     const int width = ns == 4 ? 4 : ns == 6 ? 5 : 7;
@@ -656,7 +680,9 @@ int sfo_char_face_flux_n(const double *st, const double cof[3], int scheme,
     double qL[5], qR[5], f[5];
     sfo_gemv5(R, wL, qL);
     sfo_gemv5(R, wR, qR);
-    if (sfo_face_flux(flux, qL, qR, n, gamma, f)) return SFO_ERR_FLUX;
+    const int frc = flux == SFO_LAXW ? sfo_lax_wendroff(qL, qR, n, gamma, dt, spacing, f)
+                                     : sfo_face_flux(flux, qL, qR, n, gamma, f);
+    if (frc) return SFO_ERR_FLUX;
     for (int v = 0; v < 5; ++v) out[v] = f[v] * area;
     return 0;
 }
@@ -678,6 +704,7 @@ struct sfo_grid {
     size_t cells;
     double *q, *q0, *k[4], *fx[3];
     double jac, metric[3], cof[3];
+    double dt;           /* the step's dt (Lax-Wendroff reads it) */
     unsigned err;
 };
 
@@ -721,7 +748,7 @@ sfo_grid *sfo_grid_create(const sfo_desc *desc)
     if (desc->scheme < SFO_WENO5 || desc->scheme > SFO_WENO7) return NULL;
     for (int d = 0; d < 3; ++d)
         if (desc->bc[d] < SFO_TRANSMISSIVE || desc->bc[d] > SFO_NONE) return NULL;
-    if (desc->flux < SFO_RUSANOV || desc->flux > SFO_ROE) return NULL;
+    if (desc->flux < SFO_RUSANOV || desc->flux > SFO_LAXW) return NULL;
     sfo_grid *G = (sfo_grid *)calloc(1, sizeof *G);
     if (!G) return NULL;
     G->desc = *desc;
@@ -938,8 +965,8 @@ static int compute_fluxes(sfo_grid *G, int d)
                            5 * sizeof(double));
                 }
                 double f[5];
-                int rc = sfo_char_face_flux_n(st, cof, G->desc.scheme,
-                                              G->desc.flux, G->desc.gamma, f);
+                int rc = sfo_char_face_flux_dt(st, cof, G->desc.scheme, G->desc.flux,
+                                               G->desc.gamma, G->dt, G->desc.d[d], f);
                 if (rc) { G->err |= (unsigned)rc; return -1; }
                 for (int v = 0; v < 5; ++v) {
                     if (!isfinite(f[v])) { G->err |= SFO_ERR_FLUX; return -1; }
@@ -1073,6 +1100,7 @@ double sfo_stage_abscissa(int time, int stage)
  * (SF_conservativeRHS.cpp:137-138, SF_flowOperations.cpp:648-652). */
 int sfo_step(sfo_grid *G, double dt)
 {
+    G->dt = dt;
     int p[3];
     const int scheme = G->desc.time;
     if (scheme == SFO_EULER) {

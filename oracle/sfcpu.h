@@ -29,7 +29,8 @@ extern "C" {
 #endif
 
 enum { SFO_WENO5 = 0, SFO_TENO5 = 1, SFO_WENO3 = 2, SFO_WENO7 = 3 };
-enum { SFO_RUSANOV = 0, SFO_HLLC = 1, SFO_STEGER = 2, SFO_LAXF = 3, SFO_ROE = 4 };
+enum { SFO_RUSANOV = 0, SFO_HLLC = 1, SFO_STEGER = 2, SFO_LAXF = 3, SFO_ROE = 4,
+       SFO_LAXW = 5 };
 enum { SFO_EULER = 0, SFO_SSPRK3 = 1, SFO_RK4 = 2 };
 /* per axis: zeroGradient on both ends, periodic, or an end that carries
  * no condition (the case declares none for that set): its node is advanced
@@ -72,6 +73,9 @@ int sfo_lax_friedrichs(const double qL[5], const double qR[5], const double n[3]
                        double gamma, double out[5]);
 int sfo_roe(const double qL[5], const double qR[5], const double n[3],
             double gamma, double out[5]);
+/* Lax-Wendroff: needs the step's dt and the face's node spacing */
+int sfo_lax_wendroff(const double qL[5], const double qR[5], const double n[3],
+                     double gamma, double dt, double spacing, double out[5]);
 /* any of the five face fluxes by id */
 int sfo_face_flux(int flux, const double qL[5], const double qR[5], const double n[3],
                   double gamma, double out[5]);
@@ -85,6 +89,10 @@ int sfo_char_face_flux(const double stencil[30], const double cofactor[3],
  * offsets 1-ns/2 .. ns/2 around face i+1/2 */
 int sfo_char_face_flux_n(const double *stencil, const double cofactor[3],
                          int scheme, int flux, double gamma, double out[5]);
+/* the same with the step's dt and the face's node spacing (Lax-Wendroff) */
+int sfo_char_face_flux_dt(const double *stencil, const double cofactor[3],
+                          int scheme, int flux, double gamma, double dt,
+                          double spacing, double out[5]);
 
 /* stage combination of one value, dq/dt = -r (see sfcpu.c) */
 double sfo_stage_combine(int time, int stage, double q, double q0,

@@ -15,7 +15,8 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libsonic_amd.so"
 
 SCHEMES = {"weno5": 0, "teno5": 1, "weno3": 2, "weno7": 3}
-FLUXES = {"rusanov": 0, "hllc": 1, "steger": 2, "stegerwarming": 2, "laxfriedrichs": 3, "roe": 4}
+FLUXES = {"rusanov": 0, "hllc": 1, "steger": 2, "stegerwarming": 2, "laxfriedrichs": 3, "roe": 4,
+          "laxwendroff": 5}
 TIMES = {"forwardeuler": 0, "euler": 0, "ssprk3": 1, "classicalrk4": 2, "rk4": 2}
 # per axis; "none_lo" / "none_hi" / "none": the end carries no condition
 # (its node is advanced, its ghosts keep the uploaded end-node state)

@@ -108,7 +108,7 @@ int check_desc(const sf_grid_desc* d) {
     if (!(d->gamma > 1.0) || !std::isfinite(d->gamma)) return fail(SF_E_ARG, "gamma must exceed 1");
     if (d->scheme < SF_SCHEME_WENO5 || d->scheme > SF_SCHEME_WENO7)
         return fail(SF_E_ARG, "unknown reconstruction scheme");
-    if (d->flux < SF_FLUX_RUSANOV || d->flux > SF_FLUX_ROE)
+    if (d->flux < SF_FLUX_RUSANOV || d->flux > SF_FLUX_LAX_WENDROFF)
         return fail(SF_E_ARG, "unknown face flux");
     if (d->time != SF_TIME_EULER && d->time != SF_TIME_SSPRK3 && d->time != SF_TIME_RK4)
         return fail(SF_E_ARG, "unknown time recipe");
@@ -366,6 +366,7 @@ int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
         G.m[a] = G.n[a] + 2 * G.g[a];
         G.active[a] = desc->active[a];
         G.bc[a] = desc->bc[a];
+        G.d[a] = desc->d[a];
     }
     G.stride[0] = 1;
     G.stride[1] = G.m[0];

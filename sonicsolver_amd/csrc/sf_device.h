@@ -17,6 +17,7 @@ struct GridDev {
     int active[3];
     int bc[3];          // 0 transmissive, 1 periodic
     double jac, area[3], metric[3], gamma;
+    double d[3];        // node spacing (the Lax-Wendroff flux's face spacing)
 };
 
 enum { kCopyS = 0, kAccumS = 1, kFinal = 2 };

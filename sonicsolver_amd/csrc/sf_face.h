@@ -1,7 +1,7 @@
 // sf_face.h -- per-face FP64 arithmetic of the explicit Euler hot path for
 // gfx950: characteristic WENO3 / WENO5-JS / TENO5 / WENO7 reconstruction on
 // an axis-aligned face followed by a Rusanov, HLLC, Steger-Warming,
-// Lax-Friedrichs or Roe flux.
+// Lax-Friedrichs, Roe or Lax-Wendroff flux.
 //
 // The operations are those of the reference functions cited at each
 // definition (paths relative to the sonicSolver source tree).  On an
@@ -488,6 +488,16 @@ __device__ __forceinline__ void lax_friedrichs(const double qL[5], const double 
     for (int v = 0; v < 5; ++v) out[v] = fp[v] + fm[v];
 }
 
+// physicalEulerFlux (SF_face.h:112-128) for n = e_ND.
+template <int ND>
+__device__ __forceinline__ void euler_flux(const double q[5], const Legacy& s, double f[5]) {
+    const double un = s.u[ND];
+    f[0] = q[0] * un;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) f[1 + d] = (d == ND) ? q[0] * s.u[d] * un + s.p : q[0] * s.u[d] * un;
+    f[4] = (q[4] + s.p) * un;
+}
+
 // The eigenvector matrices of Riemann::buildCharacteristicMatrix
 // (SF_riemann.cpp:69-123) at a state q for n = e_AXIS, in the factored
 // form used by the projections below.
@@ -601,34 +611,45 @@ __device__ __forceinline__ void roe(const double qL[5], const double qR[5], doub
     al[4] *= lam4;
     double dis[5];
     cb.back(al, dis);
-    // physicalEulerFlux (SF_face.h:112-128) of each side
+    double fL[5], fR[5];
+    euler_flux<ND>(qL, l, fL);
+    euler_flux<ND>(qR, r, fR);
 #pragma unroll
-    for (int side = 0; side < 2; ++side) {
-        const double* q = side ? qR : qL;
-        const Legacy& s = side ? r : l;
-        const double un = s.u[ND];
-        double f[5];
-        f[0] = q[0] * un;
+    for (int v = 0; v < 5; ++v) out[v] = 0.5 * (fL[v] + fR[v] - dis[v]);
+}
+
+// LaxWendroffFlux::flux (SF_lw.cpp:17-40): the physical flux of the
+// Richtmyer half state q* = (qL + qR)/2 - (lambda/2)(F(qR) - F(qL)),
+// lambda = dt / spacing with the step's dt.
+template <int ND>
+__device__ __forceinline__ void lax_wendroff(const double qL[5], const double qR[5],
+                                             double gamma, double lambda, double out[5],
+                                             unsigned& flags) {
+    const Legacy l = legacy_state(qL, gamma), r = legacy_state(qR, gamma);
+    if (!l.ok || !r.ok || !(lambda > 0.0)) flags |= kFlagFlux;
+    double fL[5], fR[5];
+    euler_flux<ND>(qL, l, fL);
+    euler_flux<ND>(qR, r, fR);
+    double qh[5];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) f[1 + d] = (d == ND) ? q[0] * s.u[d] * un + s.p : q[0] * s.u[d] * un;
-        f[4] = (q[4] + s.p) * un;
-#pragma unroll
-        for (int v = 0; v < 5; ++v) out[v] = side ? out[v] + f[v] : f[v];
-    }
-#pragma unroll
-    for (int v = 0; v < 5; ++v) out[v] = 0.5 * (out[v] - dis[v]);
+    for (int v = 0; v < 5; ++v) qh[v] = 0.5 * (qL[v] + qR[v]) - 0.5 * lambda * (fR[v] - fL[v]);
+    const Legacy h = legacy_state(qh, gamma);
+    if (!h.ok) flags |= kFlagFlux;
+    euler_flux<ND>(qh, h, out);
 }
 
 // Flux ids (SF_FLUX_*): 0 Rusanov, 1 HLLC, 2 Steger-Warming,
-// 3 Lax-Friedrichs, 4 Roe.
+// 3 Lax-Friedrichs, 4 Roe, 5 Lax-Wendroff (which reads lw = dt / spacing).
 template <int AXIS, int FLUX>
 __device__ __forceinline__ void riemann_flux(const double qL[5], const double qR[5],
-                                             double gamma, double out[5], unsigned& flags) {
+                                             double gamma, double lw, double out[5],
+                                             unsigned& flags) {
     constexpr int nd = Basis<AXIS>::nd;
     if constexpr (FLUX == 1) hllc<nd>(qL, qR, gamma, out);
     else if constexpr (FLUX == 2) steger_warming<nd>(qL, qR, gamma, out, flags);
     else if constexpr (FLUX == 3) lax_friedrichs<nd>(qL, qR, gamma, out, flags);
     else if constexpr (FLUX == 4) roe<AXIS>(qL, qR, gamma, out, flags);
+    else if constexpr (FLUX == 5) lax_wendroff<nd>(qL, qR, gamma, lw, out, flags);
     else rusanov<nd>(qL, qR, gamma, out);
 }
 
@@ -639,7 +660,7 @@ __device__ __forceinline__ void riemann_flux(const double qL[5], const double qR
 template <int AXIS, int SCHEME, int FLUX>
 __device__ __forceinline__ void face_flux(const double (&st)[Stencil<SCHEME>::ns][5],
                                           double gamma, double area, double out[5],
-                                          unsigned& flags) {
+                                          unsigned& flags, double lw = 0.0) {
     constexpr int NS = Stencil<SCHEME>::ns;
     // face average (SF_WENO.h:207-211) and its characteristic basis
     double qa[5];
@@ -661,7 +682,7 @@ __device__ __forceinline__ void face_flux(const double (&st)[Stencil<SCHEME>::ns
     cb.back(wL, qL);
     cb.back(wR, qR);
     double f[5];
-    riemann_flux<AXIS, FLUX>(qL, qR, gamma, f, flags);
+    riemann_flux<AXIS, FLUX>(qL, qR, gamma, lw, f, flags);
 #pragma unroll
     for (int v = 0; v < 5; ++v) out[v] = f[v] * area;
 }

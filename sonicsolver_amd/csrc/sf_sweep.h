@@ -59,7 +59,8 @@ void launch_sweep(const SweepCall& c, int flux, int axis, int mode) {
     case 1: launch_sweep_f<SCHEME, 1>(c, axis, mode); break;
     case 2: launch_sweep_f<SCHEME, 2>(c, axis, mode); break;
     case 3: launch_sweep_f<SCHEME, 3>(c, axis, mode); break;
-    default: launch_sweep_f<SCHEME, 4>(c, axis, mode); break;
+    case 4: launch_sweep_f<SCHEME, 4>(c, axis, mode); break;
+    default: launch_sweep_f<SCHEME, 5>(c, axis, mode); break;
     }
 }
 

@@ -35,6 +35,7 @@ __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) 
     const unsigned tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
     unsigned flags = 0;
     double dt_local = INFINITY;
+    const double lw = (FLUX == 5) ? *A.dt / G.d[0] : 0.0;   // Lax-Wendroff dt / spacing
     // Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
     // each with its own L2; tiles are numbered so that the blocks of one XCD
     // take consecutive tiles and share the stencil halo between them in that
@@ -56,7 +57,7 @@ __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) 
             for (int s = 0; s < NS; ++s)
 #pragma unroll
                 for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + base + (s + 1 - NS / 2)];
-            face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags);
+            face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags, lw);
 #pragma unroll
             for (int v = 0; v < 5; ++v)
                 if (!isfinite(f5[v])) flags |= kFlagFlux;
@@ -113,6 +114,7 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
     const int a1 = min(a0 + seg, G.n[AXIS]);
     unsigned flags = 0;
     double dt_local = INFINITY;
+    const double lw = (FLUX == 5) ? *A.dt / G.d[AXIS] : 0.0;   // Lax-Wendroff dt / spacing
     if (i < G.n[0]) {
         const int j0 = (AXIS == 1) ? a0 : o;
         const int k0 = (AXIS == 2) ? a0 : o;
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
             }
             head = (head == W - 1) ? 0 : head + 1;
             double f[5];
-            face_flux<AXIS, SCHEME, FLUX>(st, G.gamma, G.area[AXIS], f, flags);
+            face_flux<AXIS, SCHEME, FLUX>(st, G.gamma, G.area[AXIS], f, flags, lw);
             double s5[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {

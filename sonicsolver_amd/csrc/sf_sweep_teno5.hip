@@ -1,5 +1,5 @@
 // sf_sweep_teno5.hip -- the grid-path sweep kernels of the TENO5 reconstruction
-// (scheme id 1) for all five face fluxes.
+// (scheme id 1) for all six face fluxes.
 #include "sf_sweep.h"
 
 namespace sfamd {

@@ -15,7 +15,9 @@ FACE = {("weno5", "rusanov"): 1287, ("weno5", "hllc"): 1328, ("weno5", "steger")
         ("weno3", "rusanov"): 807, ("weno3", "hllc"): 848, ("weno3", "steger"): 948,
         ("weno3", "laxfriedrichs"): 830, ("weno3", "roe"): 1074,
         ("weno7", "rusanov"): 3327, ("weno7", "hllc"): 3368, ("weno7", "steger"): 3468,
-        ("weno7", "laxfriedrichs"): 3350, ("weno7", "roe"): 3594}
+        ("weno7", "laxfriedrichs"): 3350, ("weno7", "roe"): 3594,
+        ("weno3", "laxwendroff"): 826, ("weno5", "laxwendroff"): 1306,
+        ("teno5", "laxwendroff"): 1516, ("weno7", "laxwendroff"): 3346}
 # flux difference and Jacobian scaling, per node, by number of active axes
 RESIDUAL = {1: 20, 2: 30, 3: 40}
 # CFL scan, per node

@@ -47,6 +47,7 @@ const char* fluxName(FluxSplitter f) {
     case FluxSplitter::StegerWarming: return "Steger";
     case FluxSplitter::LaxFriedrichs: return "LaxFriedrichs";
     case FluxSplitter::Roe: return "Roe";
+    case FluxSplitter::LaxWendroff: return "LaxWendroff";
     default: return "Rusanov";
     }
 }
@@ -57,22 +58,17 @@ std::string servedList() {
     return out;
 }
 
-// Names the reference registers (toString(TermRecipeId), SF_termRecipe.h:242-258)
-// that this provider does not serve: the Lax-Wendroff recipes and the
-// primitive upwind one.
-bool registeredElsewhere(const std::string& token) {
-    static const char* schemes[] = {"weno3", "weno5", "teno5", "weno7"};
-    for (const char* sc : schemes)
-        if (token == normalizeToken(std::string(sc) + "LaxWendroff")) return true;
-    return token == "primitiveupwind1";
-}
+// The name the reference registers (toString(TermRecipeId),
+// SF_termRecipe.h:242-258) that this provider does not serve: the primitive
+// upwind recipe.
+bool registeredElsewhere(const std::string& token) { return token == "primitiveupwind1"; }
 
 }  // namespace
 
 // The reference's registration order (TermRecipeId, SF_termRecipe.h:52-62:
 // schemes WENO3, WENO5, TENO5, WENO7; fluxes Steger, Rusanov,
-// LaxFriedrichs, Roe), then the HLLC recipes, which have no reference
-// counterpart.
+// LaxFriedrichs, Roe, LaxWendroff), then the HLLC recipes, which have no
+// reference counterpart.
 const std::vector<TermRecipe>& servedConvectionRecipes() {
     static const std::vector<TermRecipe> recipes = [] {
         std::vector<TermRecipe> r;
@@ -80,7 +76,8 @@ const std::vector<TermRecipe>& servedConvectionRecipes() {
                                             ConvectionScheme::TENO5, ConvectionScheme::WENO7};
         for (ConvectionScheme sc : schemes)
             for (FluxSplitter f : {FluxSplitter::StegerWarming, FluxSplitter::Rusanov,
-                                   FluxSplitter::LaxFriedrichs, FluxSplitter::Roe})
+                                   FluxSplitter::LaxFriedrichs, FluxSplitter::Roe,
+                                   FluxSplitter::LaxWendroff})
                 r.push_back(builtInConvectionRecipe(sc, f));
         for (ConvectionScheme sc : schemes)
             r.push_back(builtInConvectionRecipe(sc, FluxSplitter::HLLC));
@@ -134,13 +131,14 @@ FluxSplitter parseFluxSplitter(const std::string& value) {
         return FluxSplitter::LaxFriedrichs;
     if (t == "roe") return FluxSplitter::Roe;
     if (t == "hllc") return FluxSplitter::HLLC;
-    if (t == "laxwendroff" || t == "lw" || t == "upwindadvection")
+    if (t == "laxwendroff" || t == "lw") return FluxSplitter::LaxWendroff;
+    if (t == "upwindadvection")
         throw std::invalid_argument("Flux method '" + value +
                                     "' is not served by the MI355X provider (StegerWarming, "
-                                    "Rusanov, LaxFriedrichs, Roe, HLLC).");
+                                    "Rusanov, LaxFriedrichs, Roe, LaxWendroff, HLLC).");
     throw std::invalid_argument("Unsupported inviscid flux method '" + value +
                                 "'. Supported methods: StegerWarming, Rusanov, LaxFriedrichs, "
-                                "Roe, HLLC.");
+                                "Roe, LaxWendroff, HLLC.");
 }
 
 // resolveTimeRecipe, SF_numericsParser.h:115-125 (empty selects classicalRK4).

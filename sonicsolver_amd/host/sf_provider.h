@@ -34,7 +34,8 @@ enum class ConvectionScheme { WENO5 = SF_SCHEME_WENO5, TENO5 = SF_SCHEME_TENO5,
                               WENO3 = SF_SCHEME_WENO3, WENO7 = SF_SCHEME_WENO7 };
 enum class FluxSplitter { Rusanov = SF_FLUX_RUSANOV, HLLC = SF_FLUX_HLLC,
                           StegerWarming = SF_FLUX_STEGER_WARMING,
-                          LaxFriedrichs = SF_FLUX_LAX_FRIEDRICHS, Roe = SF_FLUX_ROE };
+                          LaxFriedrichs = SF_FLUX_LAX_FRIEDRICHS, Roe = SF_FLUX_ROE,
+                          LaxWendroff = SF_FLUX_LAX_WENDROFF };
 
 struct TermRecipe {
     std::string id;             // canonical name, e.g. "weno5Rusanov"

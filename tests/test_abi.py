@@ -51,7 +51,7 @@ def test_invalid_descriptions_rejected_on_host():
         dict(n=(16, 1, 1), gamma=1.0),                       # gamma <= 1
         dict(n=(16, 1, 1), d=(0.0, 1.0, 1.0)),               # zero spacing
         dict(n=(16, 1, 1), scheme=7),
-        dict(n=(16, 1, 1), flux=5),
+        dict(n=(16, 1, 1), flux=6),
         dict(n=(16, 1, 1), time=9),
         dict(n=(16, 1, 1), bc=(5, 0, 0)),
         dict(n=(16, 1, 1), scheme=4),

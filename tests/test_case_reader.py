@@ -56,8 +56,8 @@ def _edit(path, old, new):
 @pytest.mark.parametrize("file,old,new,message", [
     ("solvers/numerics.yaml", "weno5Rusanov", "weno7Rusanov", "weno7"),
     ("solvers/numerics.yaml", "terms:\n  convection: weno5Rusanov",
-     "convection:\n  default: TENO5\n  flux: LaxWendroff", "LaxWendroff"),
-    ("solvers/numerics.yaml", "weno5Rusanov", "weno5LaxWendroff", "not served"),
+     "convection:\n  default: TENO5\n  flux: upwindAdvection", "upwindAdvection"),
+    ("solvers/numerics.yaml", "weno5Rusanov", "primitiveUpwind1", "not served"),
     ("fields/boundaries.yaml", "rho:\n  Left: zeroGradient\n", "rho:\n", "lower end of axis x"),
     ("fields/boundaries.yaml", "p:\n  Left: zeroGradient", "p:\n  Left: fixedValue",
      "fixedValue"),

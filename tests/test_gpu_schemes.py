@@ -1,5 +1,5 @@
-"""The WENO3 / WENO7 reconstructions and the Steger-Warming, Lax-Friedrichs
-and Roe fluxes on the HIP path, against the oracle (relative L1 / Linf
+"""The WENO3 / WENO7 reconstructions and the Steger-Warming, Lax-Friedrichs,
+Roe and Lax-Wendroff fluxes on the HIP path, against the oracle (relative L1 / Linf
 <= 1e-11 on the conserved variables), including ends that carry no
 boundary condition; and the reference's frozen WENO7 + Rusanov + forward
 Euler Sod run (tests/cases/sod_weno7_t0p2, baseline.json) through the
@@ -24,7 +24,7 @@ SOD_W7 = ROOT / "tests" / "cases" / "sod_weno7_t0p2"
 GOLDEN = ROOT / "tests" / "golden" / "sod_weno7_t0p2" / "baseline.json"
 
 SCHEMES = ["weno3", "weno5", "teno5", "weno7"]
-FLUXES = ["rusanov", "hllc", "steger", "laxfriedrichs", "roe"]
+FLUXES = ["rusanov", "hllc", "steger", "laxfriedrichs", "roe", "laxwendroff"]
 
 
 def _run_both(kw, q, steps, cfl):
@@ -78,7 +78,8 @@ def test_weno3_sod_failure_is_the_oracles():
 
 @pytest.mark.parametrize("scheme,flux,time", [
     ("weno3", "laxfriedrichs", "ssprk3"), ("weno7", "steger", "rk4"), ("weno7", "roe", "ssprk3"),
-    ("weno3", "roe", "euler"), ("teno5", "steger", "ssprk3"), ("weno5", "laxfriedrichs", "rk4")])
+    ("weno3", "roe", "euler"), ("teno5", "steger", "ssprk3"), ("weno5", "laxfriedrichs", "rk4"),
+    ("weno5", "laxwendroff", "rk4"), ("weno7", "laxwendroff", "euler")])
 def test_scheme_flux_3d_mixed_boundaries(scheme, flux, time):
     n = (22, 18, 16)
     q = wl.smooth_field(n, seed=23)

@@ -449,3 +449,29 @@ def test_an_end_without_a_condition_is_not_a_zero_gradient_end():
     q, _ = ob.run_driver(d, sod_weno7_state(), 0.0, 0.2, 0.5, write_time=0.05)
     s = sod_weno7_summary(q)
     assert abs(s["ranges"]["rho"][1] - base["ranges"]["rho"][1]) > 1e-2
+
+
+# ---- Lax-Wendroff -------------------------------------------------------------
+
+def test_lax_wendroff_consistency_and_contact_known_answer():
+    rng = np.random.default_rng(13)
+    for q in _states(rng, 30):
+        for n in NORMALS:
+            rc, f = ob.lax_wendroff(q, q, n, 0.01, 0.1)
+            _, fp = ob.physical_flux(q, n)
+            assert rc == 0 and rel(f, fp) < 1e-12
+    # a contact moving at u with uniform pressure: the Richtmyer half state
+    # is the advected density rho* = (rhoL+rhoR)/2 - (lambda u / 2)(rhoR - rhoL),
+    # lambda = dt / spacing, and the mass flux is u rho*
+    u, p, dt, h = 0.7, 1.3, 0.02, 0.1
+    qL = prim(1.0, u, 0.0, 0.0, p)
+    qR = prim(0.4, u, 0.0, 0.0, p)
+    lam = dt / h
+    rho_star = 0.5 * (1.0 + 0.4) - 0.5 * lam * u * (0.4 - 1.0)
+    rc, f = ob.lax_wendroff(qL, qR, np.array([1.0, 0.0, 0.0]), dt, h)
+    assert rc == 0
+    assert abs(f[0] - u * rho_star) < 1e-14
+    assert abs(f[1] - (u * u * rho_star + p)) < 1e-13
+    # it needs a positive dt and spacing (SF_lw.cpp:20-27)
+    assert ob.lax_wendroff(qL, qR, np.array([1.0, 0.0, 0.0]), 0.0, h)[0] != 0
+    assert ob.lax_wendroff(qL, qR, np.array([1.0, 0.0, 0.0]), dt, 0.0)[0] != 0

@@ -264,3 +264,60 @@ def test_weno7_sod_baseline_case_bit_exact():
     a, da = ob.run_driver(sod_weno7_desc(), q, 0.0, 0.2, 0.5, write_time=0.05)
     b, db = ob.ref_run_driver(sod_weno7_desc(), q, 0.0, 0.2, 0.5, write_time=0.05)
     assert np.array_equal(da, db) and np.array_equal(a, b)
+
+
+def test_lax_wendroff_bit_exact():
+    import ctypes
+    P = ctypes.POINTER(ctypes.c_double)
+    rng = np.random.default_rng(53)
+    ref = ob.ref()
+    for _ in range(100):
+        qL = wl.conserved(np.array([rng.uniform(0.1, 3)]), *rng.uniform(-2, 2, 3),
+                          rng.uniform(0.1, 5))[:, 0].copy()
+        qR = wl.conserved(np.array([rng.uniform(0.1, 3)]), *rng.uniform(-2, 2, 3),
+                          rng.uniform(0.1, 5))[:, 0].copy()
+        nv = rng.normal(size=3)
+        dt, h = rng.uniform(1e-4, 1e-2), rng.uniform(0.01, 0.2)
+        for n in (np.array([1.0, 0, 0]), np.array([0, 0, 1.0]), nv / np.linalg.norm(nv)):
+            n = np.ascontiguousarray(n)
+            rc, o = ob.lax_wendroff(qL, qR, n, dt, h)
+            if rc != 0:
+                continue
+            out = np.zeros(5)
+            assert ref.sfref_lax_wendroff(qL.ctypes.data_as(P), qR.ctypes.data_as(P),
+                                          n.ctypes.data_as(P), 1.4, dt, h,
+                                          out.ctypes.data_as(P)) == 0
+            assert np.array_equal(o, out)
+    # every scheme's characteristic face with it
+    for scheme in ("weno3", "weno5", "teno5", "weno7"):
+        ns = ob.STENCIL[ob.SCHEMES[scheme]]
+        for _ in range(20):
+            st = wl.conserved(rng.uniform(0.5, 2, ns), rng.uniform(-0.5, 0.5, ns),
+                              rng.uniform(-0.5, 0.5, ns), rng.uniform(-0.5, 0.5, ns),
+                              rng.uniform(0.5, 2, ns)).T.copy()
+            cof = np.array([0.0, 0.7, 0.0])
+            rc, o = ob.char_face_flux(st, cof, scheme, "laxwendroff", dt=0.003, spacing=0.05)
+            assert rc == 0
+            out = np.zeros(5)
+            assert ref.sfref_char_face_flux_dt(st.ctypes.data_as(P), cof.ctypes.data_as(P),
+                                               ob.SCHEMES[scheme], 5, 1.4, 0.003, 0.05,
+                                               out.ctypes.data_as(P)) == 0
+            assert np.array_equal(o, out)
+
+
+@pytest.mark.parametrize("scheme,time", [("weno5", "ssprk3"), ("weno7", "euler"),
+                                         ("teno5", "rk4"), ("weno3", "ssprk3")])
+def test_lax_wendroff_trajectory(scheme, time):
+    """The step's dt reaches the flux at every stage.  The harness takes the
+    face spacing from its node coordinates (x_i = (i - g) dx), the oracle
+    takes dx itself: they differ in the last bit, so this compares to 1e-13."""
+    from tests.parity import rel_errors
+    n = (18, 14, 1)
+    q = wl.smooth_field(n, seed=29)
+    desc = ob.make_desc(n, (1, 1, 0), ("none_lo", "periodic", "transmissive"),
+                        (0.1, 0.12, 0.09), scheme=scheme, flux="laxwendroff", time=time)
+    a, da = ob.run(desc, q, 4, 0.4)
+    b, db = ob.ref_run(desc, q, 4, 0.4)
+    np.testing.assert_allclose(da, db, rtol=1e-14)
+    l1, linf = rel_errors(a, b)
+    assert linf.max() < 1e-13

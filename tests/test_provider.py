@@ -17,6 +17,8 @@ from sonicsolver_amd import (SonicError, describe_plan, recipe_from_pair, resolv
     ("weno5Roe", (0, 4, 3)), ("teno5Steger", (1, 2, 3)), ("teno5LaxFriedrichs", (1, 3, 3)),
     ("teno5Roe", (1, 4, 3)), ("weno7Steger", (3, 2, 4)), ("weno7Rusanov", (3, 0, 4)),
     ("weno7LaxFriedrichs", (3, 3, 4)), ("weno7Roe", (3, 4, 4)),
+    ("weno3LaxWendroff", (2, 5, 2)), ("weno5LaxWendroff", (0, 5, 3)),
+    ("teno5LaxWendroff", (1, 5, 3)), ("weno7LaxWendroff", (3, 5, 4)),
     ("weno3HLLC", (2, 1, 2)), ("weno7HLLC", (3, 1, 4)),
     # the reference's token normalisation: quotes, '_', '-', spaces, case
     ("WENO5_Rusanov", (0, 0, 3)), ('"teno5-hllc"', (1, 1, 3)), (" Weno5 HLLC ", (0, 1, 3)),
@@ -25,8 +27,7 @@ def test_served_recipes_resolve(name, expected):
     assert resolve_recipe(name) == expected
 
 
-@pytest.mark.parametrize("name", ["weno3LaxWendroff", "weno5LaxWendroff", "teno5LaxWendroff",
-                                  "weno7LaxWendroff", "primitiveUpwind1"])
+@pytest.mark.parametrize("name", ["primitiveUpwind1", "PRIMITIVE_UPWIND1"])
 def test_reference_recipes_not_served_are_rejected_explicitly(name):
     with pytest.raises(SonicError) as e:
         resolve_recipe(name)
@@ -47,7 +48,9 @@ def test_legacy_pair_form():
     assert recipe_from_pair("weno7", "sw") == (3, 2, 4)
     assert recipe_from_pair("WENO3", "lxf") == (2, 3, 2)
     assert recipe_from_pair("teno5", "Roe") == (1, 4, 3)
-    for scheme, flux in (("weno5", "LaxWendroff"), ("upwind1", "rusanov"), ("weno9", "roe"),
+    assert recipe_from_pair("weno5", "LaxWendroff") == (0, 5, 3)
+    assert recipe_from_pair("weno3", "lw") == (2, 5, 2)
+    for scheme, flux in (("weno5", "upwindAdvection"), ("upwind1", "rusanov"), ("weno9", "roe"),
                          ("weno5", "godunov")):
         with pytest.raises(SonicError):
             recipe_from_pair(scheme, flux)
