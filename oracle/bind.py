@@ -145,6 +145,8 @@ def ref(fma: bool = False):
 def _declare_ref(lib):
     lib.sfref_run.argtypes = [ctypes.POINTER(Desc), _P, ctypes.c_int, ctypes.c_double,
                               ctypes.c_double, _P]
+    lib.sfref_run_slabs.argtypes = [ctypes.POINTER(Desc), _P, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, _P, _P]
     lib.sfref_residual.argtypes = [ctypes.POINTER(Desc), _P, _P]
     lib.sfref_weno5.argtypes = [_P]
     lib.sfref_weno5.restype = ctypes.c_double
@@ -261,8 +263,25 @@ def ref_run(desc: Desc, q: np.ndarray, nsteps: int, cfl: float, max_dt: float = 
     build with FMA contraction)."""
     a = _arr(q).reshape(5, -1).copy()
     dts = np.zeros(max(nsteps, 1))
-    ref(fma).sfref_run(ctypes.byref(desc), _ptr(a), nsteps, cfl, max_dt, _ptr(dts))
+    if ref(fma).sfref_run(ctypes.byref(desc), _ptr(a), nsteps, cfl, max_dt, _ptr(dts)) != 0:
+        raise RuntimeError("the reference harness raised an error")
     return a, dts[:nsteps]
+
+
+def ref_run_slabs(desc: Desc, q: np.ndarray, nsteps: int, cfl: float, nslabs: int,
+                  max_dt: float = 1e300):
+    """The reference's own step on `nslabs` threads: the grid cut into z
+    slabs (z periodic, at least the halo width of nodes each) with the
+    halo planes exchanged before every ghost fill and the dt reduced to the
+    minimum over slabs.  Returns (state, dt sequence, wall seconds of the
+    steps alone)."""
+    a = _arr(q).reshape(5, -1).copy()
+    dts = np.zeros(max(nsteps, 1))
+    sec = np.zeros(1)
+    if ref().sfref_run_slabs(ctypes.byref(desc), _ptr(a), nsteps, cfl, max_dt, nslabs,
+                             _ptr(dts), _ptr(sec)) != 0:
+        raise RuntimeError("the reference harness raised an error")
+    return a, dts[:nsteps], float(sec[0])
 
 
 def ref_run_driver(desc: Desc, q: np.ndarray, start: float, end: float, cfl: float,

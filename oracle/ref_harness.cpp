@@ -42,10 +42,15 @@
 #include "sfcpu.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -59,6 +64,9 @@ struct Case {
     SF::Field field;
     SF::Boundary::Applicator bc;
     std::vector<SF::SolverAlgorithm::PatchWorkspace> ws;
+    // A z slab of a decomposed grid: its z ghosts come from the neighbouring
+    // slabs (the exchange below) instead of the periodic fill.
+    std::function<void()> z_exchange;
 };
 
 void configure_directions(const sfo_desc& d) {
@@ -185,6 +193,7 @@ void periodic_fill(Case& c) {
     const int ng = f.NG();
     for (int a = 0; a < 3; ++a) {
         if (!c.d.active[a] || c.d.bc[a] != SFO_PERIODIC) continue;
+        if (a == 2 && c.z_exchange) continue;
         const int n = c.d.n[a];
         int lo[3] = {ng, ng, ng}, hi[3] = {ng + c.d.n[0], ng + c.d.n[1], ng + c.d.n[2]};
         int p[3];
@@ -208,6 +217,7 @@ void periodic_fill(Case& c) {
 void prepare_boundary(Case& c) {
     c.bc.apply(c.field);
     periodic_fill(c);
+    if (c.z_exchange) c.z_exchange();
     c.field.invalidateThermodynamicCache();
 }
 
@@ -548,6 +558,168 @@ int sfref_run(const sfo_desc* d, double* soa_io, int nsteps, double cfl,
         ++state.step;
     }
     export_state(*c, soa_io);
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The same step on every host core: the grid cut into z slabs, one thread
+// per slab, each slab the reference's own Field and code path.  Before every
+// ghost fill the slabs swap their ng outermost interior planes (the halo
+// exchange of the reference's parallel.split decomposition, done here in
+// shared memory) and the step's dt is the minimum over the slabs (its
+// Allreduce MIN).  The arithmetic per node is the serial run's, so the
+// result is the serial run's bit for bit.
+namespace {
+
+// A barrier that releases every waiter with an error once any slab failed.
+class SlabBarrier {
+public:
+    explicit SlabBarrier(int n) : n_(n) {}
+    void wait() {
+        std::unique_lock<std::mutex> lk(m_);
+        if (failed_) throw std::runtime_error("another slab failed");
+        const unsigned long gen = gen_;
+        if (++count_ == n_) {
+            count_ = 0;
+            ++gen_;
+            cv_.notify_all();
+            return;
+        }
+        cv_.wait(lk, [&] { return gen_ != gen || failed_; });
+        if (failed_) throw std::runtime_error("another slab failed");
+    }
+    void fail() {
+        std::lock_guard<std::mutex> lk(m_);
+        failed_ = true;
+        cv_.notify_all();
+    }
+
+private:
+    std::mutex m_;
+    std::condition_variable cv_;
+    int n_, count_ = 0;
+    unsigned long gen_ = 0;
+    bool failed_ = false;
+};
+
+}  // namespace
+
+extern "C" {
+
+// nsteps physical steps of a grid periodic in z, cut into `nslabs` z slabs
+// of at least ng nodes each, one thread per slab; `seconds` (if not null)
+// receives the wall time of the steps alone, without building the slabs.
+// Returns 0, or -1 if the description does not allow the cut or a slab
+// failed.
+int sfref_run_slabs(const sfo_desc* d, double* soa_io, int nsteps, double cfl,
+                    double max_dt, int nslabs, double* dt_seq, double* seconds) {
+  return guarded([&] {
+    const int ng = sfo_halo(d->scheme);
+    const int nz = d->n[2];
+    if (!d->active[2] || d->bc[2] != SFO_PERIODIC || nslabs < 1 || nz < nslabs * ng)
+        throw std::runtime_error("slab run needs an active periodic z axis with at least "
+                                 "ng nodes per slab");
+    const int nx = d->n[0], ny = d->n[1];
+    const size_t plane = (size_t)nx * ny, N = plane * nz;
+    std::vector<int> z0(nslabs + 1);
+    for (int s = 0; s <= nslabs; ++s) z0[s] = (int)((long long)nz * s / nslabs);
+    configure_directions(*d);
+    std::vector<std::unique_ptr<Case>> cs(nslabs);
+    for (int s = 0; s < nslabs; ++s) {
+        sfo_desc sd = *d;
+        sd.n[2] = z0[s + 1] - z0[s];
+        std::vector<double> part((size_t)5 * plane * sd.n[2]);
+        for (int v = 0; v < 5; ++v)
+            std::copy(soa_io + v * N + plane * z0[s], soa_io + v * N + plane * z0[s + 1],
+                      part.begin() + (size_t)v * plane * sd.n[2]);
+        cs[s] = make_case(sd, part.data());
+    }
+    SlabBarrier bar(nslabs);
+    std::vector<double> dts(nslabs);
+    for (int s = 0; s < nslabs; ++s) {
+        Case& c = *cs[s];
+        const Case& lo = *cs[(s + nslabs - 1) % nslabs];
+        const Case& hi = *cs[(s + 1) % nslabs];
+        c.z_exchange = [&c, &lo, &hi, &bar, ng, nx, ny] {
+            bar.wait();   // every slab has finished writing its interior
+            SF::Field& f = c.field;
+            const int nzs = c.d.n[2], nzl = lo.d.n[2];
+            for (int l = 0; l < ng; ++l)
+                for (int j = ng; j < ng + ny; ++j)
+                    for (int i = ng; i < ng + nx; ++i)
+                        for (int v = 0; v < 5; ++v) {
+                            f(i, j, l, v) = lo.field(i, j, nzl + l, v);
+                            f(i, j, ng + nzs + l, v) = hi.field(i, j, ng + l, v);
+                        }
+            bar.wait();   // every slab has read its neighbours
+        };
+    }
+    const auto recipe = time_recipe(d->time);
+    auto body = [&](int s) {
+        Case& c = *cs[s];
+        SF::State::StateBundle state;
+        state.patches = {&c.field};
+        state.stateModel = c.eq;
+        state.time = 0.0;
+        for (int step = 0; step < nsteps; ++step) {
+            prepare_boundary(c);
+            dts[s] = SF::Numerics::RusanovEOS::deltaT(c.field, *c.eq, cfl);
+            bar.wait();
+            double dt = *std::min_element(dts.begin(), dts.end());
+            dt = std::min(dt, max_dt);
+            bar.wait();   // dts is rewritten only after every slab read it
+            state.dt = dt;
+            c.dt = dt;
+            if (s == 0 && dt_seq) dt_seq[step] = dt;
+            SF::Time::Explicit::Workspace workspace;
+            SF::Time::Explicit::begin(workspace, state.patches, state, recipe, nullptr);
+            for (int st = 0; st < recipe.stageCount(); ++st) {
+                SF::Time::Explicit::executeStage(
+                    workspace, st, state.patches, c.ws, state, nullptr,
+                    [&](const std::vector<SF::Field*>&,
+                        std::vector<SF::SolverAlgorithm::PatchWorkspace>&, double) {
+                        assemble(c);
+                    },
+                    [&](const std::vector<SF::Field*>&) {},
+                    [&](const std::vector<SF::Field*>& patches, const char* label) {
+                        for (SF::Field* p : patches)
+                            SF::Numerics::validateEquationState(*p, *c.eq, label);
+                    });
+            }
+            prepare_boundary(c);
+            state.time += dt;
+            ++state.step;
+        }
+    };
+    std::vector<std::thread> th;
+    std::vector<std::string> err(nslabs);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int s = 0; s < nslabs; ++s)
+        th.emplace_back([&, s] {
+            try {
+                body(s);
+            } catch (const std::exception& e) {
+                err[s] = e.what();
+                bar.fail();
+            }
+        });
+    for (auto& t : th) t.join();
+    if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int s = 0; s < nslabs; ++s)
+        if (!err[s].empty()) throw std::runtime_error("slab " + std::to_string(s) + ": " + err[s]);
+    for (int s = 0; s < nslabs; ++s) {
+        const Case& c = *cs[s];
+        const int nzs = c.d.n[2];
+        for (int k = 0; k < nzs; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    for (int v = 0; v < 5; ++v)
+                        soa_io[v * N + ((size_t)(z0[s] + k) * ny + j) * nx + i] =
+                            c.field(i + ng, j + ng, k + ng, v);
+    }
   });
 }
 

@@ -252,7 +252,8 @@ static double energy_from_pressure(const double q[5], double gamma, double p)
 
 /* Riemann::buildCharacteristicMatrix,
  * src/methods/numerics/convection/SF_riemann.cpp:69-123.  The tangent
- * basis (l, m) is chosen exactly as there (F5 in the survey). */
+ * basis (l, m) is chosen exactly as there: l from the two components of
+ * the normal that exclude the smaller of |n2|, |n3|, and m = n x l. */
 int sfo_char_matrix(const double Q[5], const double n[3], double gamma,
                     double L[25], double R[25])
 {

@@ -321,3 +321,35 @@ def test_lax_wendroff_trajectory(scheme, time):
     np.testing.assert_allclose(da, db, rtol=1e-14)
     l1, linf = rel_errors(a, b)
     assert linf.max() < 1e-13
+
+
+@pytest.mark.parametrize("scheme,flux,time", [
+    ("weno5", "hllc", "ssprk3"), ("teno5", "roe", "rk4"), ("weno7", "laxwendroff", "ssprk3"),
+    ("weno3", "steger", "euler")])
+def test_slab_decomposed_reference_run_is_the_serial_run(scheme, flux, time):
+    """The all-cores CPU baseline: the reference's step on z slabs, one
+    thread each, with the halo planes exchanged before every ghost fill and
+    the dt reduced over slabs, gives the serial run bit for bit (uneven
+    slabs included), and the serial run is the oracle's."""
+    kw, q = wl.tgv_3d(20)
+    kw = dict(kw, n=(20, 18, 20), d=(kw["d"][0], kw["d"][0] * 20 / 18, kw["d"][0]),
+              scheme=scheme, flux=flux, time=time)
+    q = wl.smooth_field((20, 18, 20), seed=41)
+    d = ob.make_desc(**kw)
+    a, da = ob.ref_run(d, q, 3, 0.4)
+    o, do = ob.run(ob.make_desc(**kw), q, 3, 0.4)
+    assert np.array_equal(a, o.reshape(5, -1)) and np.array_equal(da, do)
+    halo = {"weno3": 2, "weno5": 3, "teno5": 3, "weno7": 4}[scheme]
+    for nslabs in (1, 3, 20 // halo):
+        b, db, sec = ob.ref_run_slabs(d, q, 3, 0.4, nslabs)
+        assert np.array_equal(a, b) and np.array_equal(da, db), nslabs
+        assert sec > 0
+
+
+def test_slab_run_rejects_what_it_cannot_cut():
+    kw, q = wl.tgv_3d(12)
+    with pytest.raises(RuntimeError):
+        ob.ref_run_slabs(ob.make_desc(**kw), q, 1, 0.4, 5)          # 12 < 5 slabs x 3 nodes
+    kw2 = dict(kw, bc=("periodic", "periodic", "transmissive"))
+    with pytest.raises(RuntimeError):
+        ob.ref_run_slabs(ob.make_desc(**kw2), q, 1, 0.4, 2)         # z not periodic
