@@ -163,8 +163,9 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
             A.stage = st.stage;
             A.reduce_dt = st.reduce_dt;
             const ItemView G = item_view(B, it);
-            if (item_solved(it, a, j)) stage_update(G, A, id, q, s5, flags, dl);
-            else carry_node(G, A, id, q);
+            double qn[5];
+            if (item_solved(it, a, j)) stage_update(G, A, id, q, s5, flags, dl, qn);
+            else carry_node(G, A, id, q, qn);
         }
         if (st.reduce_dt) block_min_atomic(dl, &B.dtmin[T.x], red);
         if (flags) {

@@ -144,8 +144,10 @@ void prof_stop(sf_grid* g, int idx) {
 
 // Ghost fill in the order of Boundary::Applicator::apply (density and
 // momentum for every transmissive set, then energy from pressure, axis by
-// axis), then the periodic axes.
-int fill_ghosts(sf_grid* g, double* q) {
+// axis), then the periodic axes.  `all` = 0 skips the periodic axes whose
+// ghosts the final sweep keeps as copies of the interior (G.images): a
+// buffer written by a stage needs no fill for them.
+int fill_ghosts(sf_grid* g, double* q, int all) {
     const GridDev& G = g->G;
     const int ev = prof_start(g, SF_KERNEL_GHOST);
     auto plane = [&](int a) {
@@ -170,7 +172,8 @@ int fill_ghosts(sf_grid* g, double* q) {
         if (a == 2) k_bc_energy<2><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
     }
     for (int a = 0; a < 3; ++a) {
-        if (!G.active[a] || G.bc[a] != 1) continue;
+        if (!G.active[a] || G.bc[a] != SF_BC_PERIODIC) continue;
+        if (!all && (G.images & (1 << a))) continue;
         const unsigned nb = blocks_for(plane(a), 256);
         if (a == 0) k_bc_periodic<0><<<nb, 256, 0, g->stream>>>(G, q);
         if (a == 1) k_bc_periodic<1><<<nb, 256, 0, g->stream>>>(G, q);
@@ -262,10 +265,10 @@ int set_limits(sf_grid* g, double max_dt, double limit) {
 int do_stage(sf_grid* g, int s, int fold_dt) {
     const int in = g->stage_in[s], out = g->stage_out[s];
     if (s == 0 && !g->ghosts_valid) {
-        if (int rc = fill_ghosts(g, g->buf[in])) return rc;
+        if (int rc = fill_ghosts(g, g->buf[in], 1)) return rc;
     }
     if (s > 0) {
-        if (int rc = fill_ghosts(g, g->buf[in])) return rc;
+        if (int rc = fill_ghosts(g, g->buf[in], 0)) return rc;
     }
     StageArgs A{};
     A.q0 = g->buf[g->cur];
@@ -367,7 +370,15 @@ int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
         G.active[a] = desc->active[a];
         G.bc[a] = desc->bc[a];
         G.d[a] = desc->d[a];
+        G.images |= (desc->active[a] && desc->bc[a] == SF_BC_PERIODIC) ? (1 << a) : 0;
     }
+    // A zeroGradient end rewrites the boundary nodes of its axis in the
+    // ghost fill after the stage; copies written by the stage would miss
+    // that, so the copies are used only where no axis has such an end.
+    for (int a = 0; a < 3; ++a)
+        if (desc->active[a] && (desc->bc[a] == SF_BC_TRANSMISSIVE ||
+                                desc->bc[a] == SF_BC_NONE_LO || desc->bc[a] == SF_BC_NONE_HI))
+            G.images = 0;
     G.stride[0] = 1;
     G.stride[1] = G.m[0];
     G.stride[2] = (long long)G.m[0] * G.m[1];
@@ -521,7 +532,7 @@ int sf_step_prepare(sf_grid* g) {
     if (int rc = set_device(g)) return rc;
     if (g->step_open) return fail(SF_E_STATE, "sf_step_prepare: a step is open");
     if (!g->ghosts_valid) {
-        if (int rc = fill_ghosts(g, g->buf[g->cur])) return rc;
+        if (int rc = fill_ghosts(g, g->buf[g->cur], 1)) return rc;
         g->ghosts_valid = 1;
     }
     return 0;
@@ -589,7 +600,7 @@ int sf_step_commit(sf_grid* g) {
         return fail(SF_E_STATE, "sf_step_commit: stages incomplete");
     if (int rc = set_device(g)) return rc;
     g->cur = g->stage_out[nstages(g->desc.time) - 1];
-    if (int rc = fill_ghosts(g, g->buf[g->cur])) return rc;
+    if (int rc = fill_ghosts(g, g->buf[g->cur], 0)) return rc;
     k_commit_clock<<<1, 64, 0, g->stream>>>(g->clock);
     SF_HIP(hipGetLastError());
     g->ghosts_valid = 1;

@@ -15,9 +15,13 @@ struct GridDev {
     long long P;        // padded nodes per variable
     long long stride[3];
     int active[3];
-    int bc[3];          // 0 transmissive, 1 periodic
+    int bc[3];          // SF_BC_* kind of each axis
     double jac, area[3], metric[3], gamma;
     double d[3];        // node spacing (the Lax-Wendroff flux's face spacing)
+    // Bit a set: the final sweep writes every node it updates also into the
+    // periodic ghost positions of axis a that hold its copies, so no ghost
+    // fill is needed for that axis between stages.
+    int images;
 };
 
 enum { kCopyS = 0, kAccumS = 1, kFinal = 2 };
@@ -94,15 +98,39 @@ __device__ __forceinline__ void raise_flags(unsigned f, unsigned* flags) {
     if (f) atomicOr(flags, f);
 }
 
+// Node (i, j, k) at `id` was just written with qn: copy it into the ghost
+// positions of the axes in G.images that hold its periodic copies (node c
+// of an axis with n nodes and g ghost layers is ghost c + n if c < g and
+// ghost c - n if c >= n - g; n >= 4 >= g, so there is at most one copy per
+// axis).  The sweeps read ghosts only along their own axis, so no corner
+// copies are needed.
+__device__ __forceinline__ void write_images(const GridDev& G, double* __restrict__ out, int i,
+                                             int j, int k, long long id, const double qn[5]) {
+    const int c[3] = {i, j, k};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!(G.images & (1 << a))) continue;
+        long long off;
+        if (c[a] < G.g[a]) off = (long long)G.n[a] * G.stride[a];
+        else if (c[a] >= G.n[a] - G.g[a]) off = -(long long)G.n[a] * G.stride[a];
+        else continue;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) out[v * G.P + id + off] = qn[v];
+    }
+}
+
 // The reference updates the state in place, so a node that the stage does
 // not advance (a solver-boundary node) keeps the value of the last ghost
 // fill; with separate input and output buffers it is copied across.
 template <class GT>
 __device__ __forceinline__ void carry_node(const GT& G, const StageArgs& A, long long id,
-                                           const double* __restrict__ q) {
+                                           const double* __restrict__ q, double qn[5]) {
     if (A.time == 3) return;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) A.qout[v * G.P + id] = q[v * G.P + id];
+    for (int v = 0; v < 5; ++v) {
+        qn[v] = q[v * G.P + id];
+        A.qout[v * G.P + id] = qn[v];
+    }
 }
 
 // Stage update of one node; S is the accumulated flux difference.  GT is
@@ -111,14 +139,13 @@ __device__ __forceinline__ void carry_node(const GT& G, const StageArgs& A, long
 template <class GT>
 __device__ __forceinline__ void stage_update(const GT& G, const StageArgs& A, long long id,
                                              const double* __restrict__ q, const double S[5],
-                                             unsigned& flags, double& dt_local) {
+                                             unsigned& flags, double& dt_local, double qn[5]) {
     if (A.time == 3) {  // residual only (diagnostics)
 #pragma unroll
         for (int v = 0; v < 5; ++v) A.qout[v * G.P + id] = G.jac * S[v];
         return;
     }
     const double dt = *A.dt;
-    double qn[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         const double R = G.jac * S[v];

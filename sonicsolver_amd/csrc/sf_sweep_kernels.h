@@ -75,8 +75,10 @@ __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) 
             }
             if (MODE == kFinal) {
                 double dl = INFINITY;
-                if (solved_node(G, a, j, k)) stage_update(G, A, id, q, s5, flags, dl);
-                else carry_node(G, A, id, q);
+                double qn[5];
+                if (solved_node(G, a, j, k)) stage_update(G, A, id, q, s5, flags, dl, qn);
+                else carry_node(G, A, id, q, qn);
+                if (G.images && A.time != 3) write_images(G, A.qout, a, j, k, id, qn);
                 dt_local = fmin(dt_local, dl);
             } else {
 #pragma unroll
@@ -171,8 +173,10 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
                 const int jj = (AXIS == 1) ? a : o;
                 const int kk = (AXIS == 2) ? a : o;
                 double dl = INFINITY;
-                if (solved_node(G, i, jj, kk)) stage_update(G, A, id, q, s5, flags, dl);
-                else carry_node(G, A, id, q);
+                double qn[5];
+                if (solved_node(G, i, jj, kk)) stage_update(G, A, id, q, s5, flags, dl, qn);
+                else carry_node(G, A, id, q, qn);
+                if (G.images && A.time != 3) write_images(G, A.qout, i, jj, kk, id, qn);
                 dt_local = fmin(dt_local, dl);
             } else {
 #pragma unroll
