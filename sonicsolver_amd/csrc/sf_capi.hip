@@ -379,18 +379,30 @@ int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
         if (desc->active[a] && (desc->bc[a] == SF_BC_TRANSMISSIVE ||
                                 desc->bc[a] == SF_BC_NONE_LO || desc->bc[a] == SF_BC_NONE_HI))
             G.images = 0;
+    for (int a = 0; a < 3; ++a) {
+        G.solved_lo[a] = zg_end(G, a, 0) ? 1 : 0;
+        G.solved_hi[a] = G.n[a] - 1 - (zg_end(G, a, 1) ? 1 : 0);
+    }
     G.stride[0] = 1;
     G.stride[1] = G.m[0];
     G.stride[2] = (long long)G.m[0] * G.m[1];
     G.P = (long long)G.m[0] * G.m[1] * G.m[2];
+    for (int a = 0; a < 3; ++a) {
+        const bool img = (G.images >> a) & 1;
+        G.img_lo[a] = img ? G.g[a] : -1;
+        G.img_hi[a] = img ? G.n[a] - G.g[a] : std::numeric_limits<int>::max();
+        G.img_off[a] = img ? (long long)G.n[a] * G.stride[a] : 0;
+    }
     G.gamma = desc->gamma;
     // Uniform Cartesian metrics exactly as the reference forms them:
     // Jac = 1/J, metric = 1/d, face cofactor = 0.5*(metric/Jac + metric/Jac),
     // area = |cofactor| (SF_mesh.cpp:320-338, SF_canonicalFace.h:87-108).
+    // The metric of an inactive axis is stored as 0: its CFL term, which
+    // the reference skips, then adds an exact zero.
     G.jac = 1.0 / (desc->d[0] * desc->d[1] * desc->d[2]);
     for (int a = 0; a < 3; ++a) {
-        G.metric[a] = 1.0 / desc->d[a];
-        const double c = G.metric[a] / G.jac;
+        G.metric[a] = desc->active[a] ? 1.0 / desc->d[a] : 0.0;
+        const double c = (1.0 / desc->d[a]) / G.jac;
         const double cof = 0.5 * (c + c);
         G.area[a] = std::sqrt(cof * cof + 0.0 * 0.0 + 0.0 * 0.0);
     }

@@ -16,12 +16,21 @@ struct GridDev {
     long long stride[3];
     int active[3];
     int bc[3];          // SF_BC_* kind of each axis
-    double jac, area[3], metric[3], gamma;
+    double jac, area[3], gamma;
+    double metric[3];   // 1 / spacing on active axes, 0 on inactive ones
     double d[3];        // node spacing (the Lax-Wendroff flux's face spacing)
     // Bit a set: the final sweep writes every node it updates also into the
     // periodic ghost positions of axis a that hold its copies, so no ghost
-    // fill is needed for that axis between stages.
+    // fill is needed for that axis between stages.  A node c of such an axis
+    // has a copy at c + n if c < img_lo[a] and at c - n if c >= img_hi[a]
+    // (img_lo = g, img_hi = n - g, img_off = n * stride); on other axes
+    // img_lo = -1 and img_hi = INT_MAX, so no node has a copy.
     int images;
+    int img_lo[3], img_hi[3];
+    long long img_off[3];
+    // Nodes the stage advances along each axis: solved_lo[a] .. solved_hi[a]
+    // (a zeroGradient end's node is excluded; see solved_node).
+    int solved_lo[3], solved_hi[3];
 };
 
 enum { kCopyS = 0, kAccumS = 1, kFinal = 2 };
@@ -58,15 +67,17 @@ __host__ __device__ __forceinline__ bool zg_end(const GridDev& G, int d, int sid
 // A node is advanced unless it lies on a zeroGradient end, which the
 // boundary applicator marks as a solver-boundary point
 // (SF_applicator.h:162-187); an end without a condition is not marked.
+// The host folds that rule into the per-axis ranges solved_lo / solved_hi.
 __device__ __forceinline__ bool solved_node(const GridDev& G, int i, int j, int k) {
-    const int p[3] = {i, j, k};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        if (zg_end(G, d, 0) && p[d] == 0) return false;
-        if (zg_end(G, d, 1) && p[d] == G.n[d] - 1) return false;
-    }
-    return true;
+    return i >= G.solved_lo[0] && i <= G.solved_hi[0] && j >= G.solved_lo[1] &&
+           j <= G.solved_hi[1] && k >= G.solved_lo[2] && k <= G.solved_hi[2];
 }
+
+// Whether the CFL sum takes axis d: a grid's metric is zero on an inactive
+// axis, so its term adds an exact zero and needs no test.
+template <class GT>
+__device__ __forceinline__ bool cfl_axis(const GT& G, int d) { return G.active[d]; }
+__device__ __forceinline__ bool cfl_axis(const GridDev&, int) { return true; }
 
 // Local CFL step of RusanovEOS::deltaT (SF_rusanovEOS.cpp:110-147).  GT is
 // any descriptor with gamma, active[3] and metric[3].
@@ -77,7 +88,7 @@ __device__ __forceinline__ double local_dt(const GT& G, const double q[5], doubl
     double spectral = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-        if (G.active[d]) spectral += fabs(s.u[d] * G.metric[d]) + s.c * G.metric[d];
+        if (cfl_axis(G, d)) spectral += fabs(s.u[d] * G.metric[d]) + s.c * G.metric[d];
     if (!(spectral > 0.0) || !isfinite(spectral)) flags |= kFlagDt;
     return cfl / spectral;
 }
@@ -109,10 +120,9 @@ __device__ __forceinline__ void write_images(const GridDev& G, double* __restric
     const int c[3] = {i, j, k};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        if (!(G.images & (1 << a))) continue;
         long long off;
-        if (c[a] < G.g[a]) off = (long long)G.n[a] * G.stride[a];
-        else if (c[a] >= G.n[a] - G.g[a]) off = -(long long)G.n[a] * G.stride[a];
+        if (c[a] < G.img_lo[a]) off = G.img_off[a];
+        else if (c[a] >= G.img_hi[a]) off = -G.img_off[a];
         else continue;
 #pragma unroll
         for (int v = 0; v < 5; ++v) out[v * G.P + id + off] = qn[v];
@@ -122,10 +132,11 @@ __device__ __forceinline__ void write_images(const GridDev& G, double* __restric
 // The reference updates the state in place, so a node that the stage does
 // not advance (a solver-boundary node) keeps the value of the last ghost
 // fill; with separate input and output buffers it is copied across.
-template <class GT>
+template <int TIME = -1, class GT>
 __device__ __forceinline__ void carry_node(const GT& G, const StageArgs& A, long long id,
                                            const double* __restrict__ q, double qn[5]) {
-    if (A.time == 3) return;
+    const int time = TIME >= 0 ? TIME : A.time;
+    if (time == 3) return;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         qn[v] = q[v * G.P + id];
@@ -135,12 +146,14 @@ __device__ __forceinline__ void carry_node(const GT& G, const StageArgs& A, long
 
 // Stage update of one node; S is the accumulated flux difference.  GT is
 // any descriptor with P, jac, gamma, active[3] and metric[3]; the step's dt
-// is read from *A.dt.
-template <class GT>
+// is read from *A.dt.  TIME fixes the time recipe (A.time) at compile time;
+// -1 reads it from A.
+template <int TIME = -1, class GT>
 __device__ __forceinline__ void stage_update(const GT& G, const StageArgs& A, long long id,
                                              const double* __restrict__ q, const double S[5],
                                              unsigned& flags, double& dt_local, double qn[5]) {
-    if (A.time == 3) {  // residual only (diagnostics)
+    const int time = TIME >= 0 ? TIME : A.time;
+    if (time == 3) {  // residual only (diagnostics)
 #pragma unroll
         for (int v = 0; v < 5; ++v) A.qout[v * G.P + id] = G.jac * S[v];
         return;
@@ -150,10 +163,10 @@ __device__ __forceinline__ void stage_update(const GT& G, const StageArgs& A, lo
     for (int v = 0; v < 5; ++v) {
         const double R = G.jac * S[v];
         const double qc = q[v * G.P + id];
-        if (A.time == 1) {
+        if (time == 1) {
             const double euler = qc - dt * R;
             qn[v] = (A.stage == 0 ? 0.0 : A.base * A.q0[v * G.P + id]) + A.incr * euler;
-        } else if (A.time == 0) {
+        } else if (time == 0) {
             qn[v] = qc - dt * R;
         } else if (A.stage < 3) {
             A.k[A.stage][v * G.P + id] = R;

@@ -23,7 +23,9 @@ constexpr int kXBlock = 64;
 // The final HLLC sweeps fit four waves per SIMD (128 VGPRs) without
 // spilling when asked to; the others stay at three (the final Rusanov sweeps
 // would spill at four).
-template <int SCHEME, int FLUX, int MODE>
+// TIME: the time recipe of a final sweep (A.time, fixed at compile time so
+// that the update carries no code for the other recipes); 0 otherwise.
+template <int SCHEME, int FLUX, int MODE, int TIME>
 __global__ __launch_bounds__(kXBlock)
 __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) ? 4 : 3, 8))) void k_sweep_x(GridDev G, const double* __restrict__ q,
                                                      double* __restrict__ S, StageArgs A) {
@@ -76,9 +78,9 @@ __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) 
             if (MODE == kFinal) {
                 double dl = INFINITY;
                 double qn[5];
-                if (solved_node(G, a, j, k)) stage_update(G, A, id, q, s5, flags, dl, qn);
-                else carry_node(G, A, id, q, qn);
-                if (G.images && A.time != 3) write_images(G, A.qout, a, j, k, id, qn);
+                if (solved_node(G, a, j, k)) stage_update<TIME>(G, A, id, q, s5, flags, dl, qn);
+                else carry_node<TIME>(G, A, id, q, qn);
+                if (TIME != 3 && G.images) write_images(G, A.qout, a, j, k, id, qn);
                 dt_local = fmin(dt_local, dl);
             } else {
 #pragma unroll
