@@ -523,8 +523,8 @@ BatchDev view(const sf_batch* b) {
 }
 
 void drop_graph(sf_batch* b) {
-    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     b->graph_exec = nullptr;
     b->graph = nullptr;
 }
@@ -814,8 +814,9 @@ int sf_batch_create(int device, double gamma, size_t node_capacity, int item_cap
     b->gamma = gamma;
     b->capacity = (long long)node_capacity;
     b->item_capacity = item_capacity;
-    hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device);
-    if (b->cus <= 0) b->cus = 256;
+    if (hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess || b->cus <= 0)
+        b->cus = 256;   // MI355X's CU count; only the launch shapes depend on it
     const size_t state = sizeof(double) * 5 * node_capacity;
     bool ok = true;
     for (int i = 0; i < 3 && ok; ++i) ok = hipMalloc(&b->buf[i], state) == hipSuccess;
@@ -853,21 +854,21 @@ int sf_batch_create(int device, double gamma, size_t node_capacity, int item_cap
 
 int sf_batch_destroy(sf_batch* b) {
     if (!b) return 0;
-    hipSetDevice(b->device);
-    if (b->stream) hipStreamSynchronize(b->stream);
-    if (b->copy_stream) hipStreamSynchronize(b->copy_stream);
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);
     drop_graph(b);
     for (double* p : b->buf)
-        if (p) hipFree(p);
-    if (b->S) hipFree(b->S);
-    if (b->staging) hipFree(b->staging);
+        if (p) (void)hipFree(p);
+    if (b->S) (void)hipFree(b->S);
+    if (b->staging) (void)hipFree(b->staging);
     void* ptrs[] = {b->d_items, b->d_dt, b->d_dtmin, b->d_time,   b->d_steps, b->d_target,
                     b->d_flags, b->d_xt, b->d_yt,    b->d_active, b->d_scan};
     for (void* p : ptrs)
-        if (p) hipFree(p);
-    if (b->stream) hipStreamDestroy(b->stream);
-    if (b->copy_stream) hipStreamDestroy(b->copy_stream);
-    if (b->copy_after) hipEventDestroy(b->copy_after);
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
+    if (b->copy_after) (void)hipEventDestroy(b->copy_after);
     delete b;
     return 0;
 }
@@ -1138,8 +1139,8 @@ int time_shape(sf_batch* b, int steps, double* ms) {
     float t = 0.0f;
     if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess)
         rc = bfail(SF_E_HIP, "event timing failed");
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     *ms = (double)t / steps;
     return rc;
 }
@@ -1185,7 +1186,7 @@ extern "C" int sf_batch_autotune(sf_batch* b, int trial_steps, sf_batch_tuning* 
                         best.ms_per_step = ms;
                     }
                 }
-    if (sn.q) hipFree(sn.q);
+    if (sn.q) (void)hipFree(sn.q);
     if (rc) {
         b->xbs = keep[0];
         b->yseg = keep[1];

@@ -134,12 +134,12 @@ int prof_start(sf_grid* g, int kernel) {
     }
     const int idx = (int)(2 * g->ev_used.size());
     g->ev_used.push_back({kernel, idx});
-    hipEventRecord(g->ev_pool[idx], g->stream);
+    (void)hipEventRecord(g->ev_pool[idx], g->stream);
     return idx;
 }
 
 void prof_stop(sf_grid* g, int idx) {
-    if (idx >= 0) hipEventRecord(g->ev_pool[idx + 1], g->stream);
+    if (idx >= 0) (void)hipEventRecord(g->ev_pool[idx + 1], g->stream);
 }
 
 // Ghost fill in the order of Boundary::Applicator::apply (density and
@@ -317,16 +317,16 @@ void plan_buffers(sf_grid* g) {
 }
 
 int grid_free(sf_grid* g) {
-    if (g->stream) hipStreamSynchronize(g->stream);
-    for (double* p : g->buf) if (p) hipFree(p);
-    for (double* p : g->k) if (p) hipFree(p);
-    if (g->S) hipFree(g->S);
-    if (g->clock) hipFree(g->clock);
-    if (g->flags) hipFree(g->flags);
-    if (g->dt_hist) hipFree(g->dt_hist);
-    if (g->stage_io) hipFree(g->stage_io);
-    for (hipEvent_t e : g->ev_pool) hipEventDestroy(e);
-    if (g->stream) hipStreamDestroy(g->stream);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (double* p : g->buf) if (p) (void)hipFree(p);
+    for (double* p : g->k) if (p) (void)hipFree(p);
+    if (g->S) (void)hipFree(g->S);
+    if (g->clock) (void)hipFree(g->clock);
+    if (g->flags) (void)hipFree(g->flags);
+    if (g->dt_hist) (void)hipFree(g->dt_hist);
+    if (g->stage_io) (void)hipFree(g->stage_io);
+    for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return 0;
 }

@@ -408,8 +408,8 @@ int sf_server_destroy(sf_server* s) {
     if (s->worker.joinable()) s->worker.join();
     sf_batch_destroy(s->batch);
     for (auto& b : s->pinned)
-        if (b.p) hipHostFree(b.p);
-    for (hipEvent_t e : s->events) hipEventDestroy(e);
+        if (b.p) (void)hipHostFree(b.p);
+    for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
     delete s;
     return 0;
 }
