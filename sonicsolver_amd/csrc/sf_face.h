@@ -116,21 +116,25 @@ struct PairTerms {
     double bL[3], bR[3];         // smoothness indicators
 };
 
+// The indicators come out as beta + eps: eps enters as the addend of the
+// innermost fused multiply-add of each, so the weights' eps + beta costs no
+// separate addition (eps = 0 gives beta itself).
 __device__ __forceinline__ PairTerms pair_terms(double c0, double c1, double c2, double c3,
-                                                double c4, double c5) {
+                                                double c4, double c5, double eps) {
     PairTerms t;
     t.qL0 = 2.0 * c0 - 7.0 * c1 + 11.0 * c2;
     t.qR0 = 2.0 * c5 - 7.0 * c4 + 11.0 * c3;
     t.Sa = -c1 + 5.0 * c2 + 2.0 * c3;   // left S1 = right S2
     t.Sb = 2.0 * c2 + 5.0 * c3 - c4;    // left S2 = right S1
+    constexpr double k = 13.0 / 12.0;
     const double dB = sq(c1 - 2.0 * c2 + c3);
     const double dC = sq(c2 - 2.0 * c3 + c4);
-    t.bL[0] = 13.0 / 12.0 * sq(c0 - 2.0 * c1 + c2) + 0.25 * sq(c0 - 4.0 * c1 + 3.0 * c2);
-    t.bL[1] = 13.0 / 12.0 * dB + 0.25 * sq(c1 - c3);
-    t.bL[2] = 13.0 / 12.0 * dC + 0.25 * sq(3.0 * c2 - 4.0 * c3 + c4);
-    t.bR[0] = 13.0 / 12.0 * sq(c5 - 2.0 * c4 + c3) + 0.25 * sq(c5 - 4.0 * c4 + 3.0 * c3);
-    t.bR[1] = 13.0 / 12.0 * dC + 0.25 * sq(c4 - c2);
-    t.bR[2] = 13.0 / 12.0 * dB + 0.25 * sq(3.0 * c3 - 4.0 * c2 + c1);
+    t.bL[0] = fma(k, sq(c0 - 2.0 * c1 + c2), fma(0.25, sq(c0 - 4.0 * c1 + 3.0 * c2), eps));
+    t.bL[1] = fma(0.25, sq(c1 - c3), fma(k, dB, eps));
+    t.bL[2] = fma(0.25, sq(3.0 * c2 - 4.0 * c3 + c4), fma(k, dC, eps));
+    t.bR[0] = fma(k, sq(c5 - 2.0 * c4 + c3), fma(0.25, sq(c5 - 4.0 * c4 + 3.0 * c3), eps));
+    t.bR[1] = fma(0.25, sq(c4 - c2), fma(k, dC, eps));
+    t.bR[2] = fma(0.25, sq(3.0 * c3 - 4.0 * c2 + c1), fma(k, dB, eps));
     return t;
 }
 
@@ -146,15 +150,18 @@ __device__ __forceinline__ void div_pair(double a, double b, double c, double d,
 __device__ __forceinline__ void weno5_pair(double c0, double c1, double c2, double c3,
                                            double c4, double c5, double& left,
                                            double& right) {
-    const PairTerms t = pair_terms(c0, c1, c2, c3, c4, c5);
-    const double eps = 1e-6;
-    const double eL0 = sq(eps + t.bL[0]), eL1 = sq(eps + t.bL[1]), eL2 = sq(eps + t.bL[2]);
-    const double eR0 = sq(eps + t.bR[0]), eR1 = sq(eps + t.bR[1]), eR2 = sq(eps + t.bR[2]);
-    const double wL0 = 0.1 * (eL1 * eL2), wL1 = 0.6 * (eL0 * eL2), wL2 = 0.3 * (eL0 * eL1);
-    const double wR0 = 0.1 * (eR1 * eR2), wR1 = 0.6 * (eR0 * eR2), wR2 = 0.3 * (eR0 * eR1);
-    const double nL = wL0 * t.qL0 + wL1 * t.Sa + wL2 * t.Sb;
-    const double nR = wR0 * t.qR0 + wR1 * t.Sb + wR2 * t.Sa;
-    div_pair(nL, 6.0 * (wL0 + wL1 + wL2), nR, 6.0 * (wR0 + wR1 + wR2), left, right);
+    const PairTerms t = pair_terms(c0, c1, c2, c3, c4, c5, 1e-6);   // beta + eps
+    const double eL0 = sq(t.bL[0]), eL1 = sq(t.bL[1]), eL2 = sq(t.bL[2]);
+    const double eR0 = sq(t.bR[0]), eR1 = sq(t.bR[1]), eR2 = sq(t.bR[2]);
+    // alpha_k e0 e1 e2 / 10 with d = (0.1, 0.6, 0.3): (1, 6, 3) x the
+    // products of the other two e; the candidates carry a factor 6, so the
+    // denominator is 6 x (A + 6B + 3C)
+    const double AL = eL1 * eL2, BL = eL0 * eL2, CL = eL0 * eL1;
+    const double AR = eR1 * eR2, BR = eR0 * eR2, CR = eR0 * eR1;
+    const double nL = AL * t.qL0 + (6.0 * BL) * t.Sa + (3.0 * CL) * t.Sb;
+    const double nR = AR * t.qR0 + (6.0 * BR) * t.Sb + (3.0 * CR) * t.Sa;
+    div_pair(nL, 6.0 * AL + 36.0 * BL + 18.0 * CL, nR, 6.0 * AR + 36.0 * BR + 18.0 * CR, left,
+             right);
 }
 
 __device__ __forceinline__ void teno5_weights(const double b[3], double& c0, double& c1,
@@ -172,7 +179,7 @@ __device__ __forceinline__ void teno5_weights(const double b[3], double& c0, dou
 __device__ __forceinline__ void teno5_pair(double c0, double c1, double c2, double c3,
                                            double c4, double c5, double& left, double& right,
                                            unsigned& flags) {
-    const PairTerms t = pair_terms(c0, c1, c2, c3, c4, c5);
+    const PairTerms t = pair_terms(c0, c1, c2, c3, c4, c5, 0.0);
     double l0, l1, l2, r0, r1, r2;
     teno5_weights(t.bL, l0, l1, l2);
     teno5_weights(t.bR, r0, r1, r2);
@@ -195,8 +202,8 @@ __device__ __forceinline__ void weno3_pair(const double c[4], double& left, doub
     const double A = -c[0] + 5.0 * c[1] + 2.0 * c[2];   // 6 x left S0 = right S1
     const double B = 2.0 * c[1] + 5.0 * c[2] - c[3];    // 6 x left S1 = right S0
     const double eps = 1e-6;
-    const double eA = sq(eps + sq(c[2] - c[0]));
-    const double eB = sq(eps + sq(c[3] - c[1]));
+    const double eA = sq(fma(c[2] - c[0], c[2] - c[0], eps));
+    const double eB = sq(fma(c[3] - c[1], c[3] - c[1], eps));
     const double d0 = 1.0 / 3.0, d1 = 2.0 / 3.0;
     const double nL = d0 * eB * A + d1 * eA * B, nR = d0 * eA * B + d1 * eB * A;
     const double sL = d0 * eB + d1 * eA, sR = d0 * eA + d1 * eB;
@@ -219,8 +226,9 @@ constexpr W7Interval w7_interval(double a, double b) {
             b * b * b * b * b / 5.0 - a * a * a * a * a / 5.0};
 }
 
+// Returns beta + eps, eps added inside the last fused multiply-add.
 __device__ __forceinline__ double w7_beta(double v0, double v1, double v2, double v3,
-                                          const W7Interval& I) {
+                                          const W7Interval& I, double eps) {
     const double d1 = v1 - v0;
     const double d2 = (v2 - 2.0 * v1 + v0) * 0.5;
     const double d3 = (v3 - 3.0 * v2 + 3.0 * v1 - v0) * (1.0 / 6.0);
@@ -231,7 +239,7 @@ __device__ __forceinline__ double w7_beta(double v0, double v1, double v2, doubl
     const double beta1 = A * A + A * B * I.i2 + (B * B + 2.0 * A * C) * I.i3 + B * C * I.i4 +
                          C * C * I.i5;
     const double beta2 = B * B + B * E * I.i2 + E * E * I.i3;
-    const double beta3 = 36.0 * d3 * d3;
+    const double beta3 = fma(36.0 * d3, d3, eps);
     return beta1 + beta2 + beta3;
 }
 
@@ -246,9 +254,9 @@ __device__ __forceinline__ double w7_beta(double v0, double v1, double v2, doubl
 // product of the four e_j) and the two divisions share one reciprocal.
 __device__ __forceinline__ void weno7_side_weights(const double b[4], double w[4]) {
     constexpr double d0 = 1.0 / 35.0, d1 = 12.0 / 35.0, d2 = 18.0 / 35.0, d3 = 4.0 / 35.0;
-    const double eps = 1e-6;
-    const double e0 = sq(eps + b[0]), e1 = sq(eps + b[1]);
-    const double e2 = sq(eps + b[2]), e3 = sq(eps + b[3]);
+    // b[k] = beta_k + eps (w7_beta)
+    const double e0 = sq(b[0]), e1 = sq(b[1]);
+    const double e2 = sq(b[2]), e3 = sq(b[3]);
     const double e01 = e0 * e1, e23 = e2 * e3;
     w[0] = d0 * (e1 * e23);
     w[1] = d1 * (e0 * e23);
@@ -266,15 +274,16 @@ __device__ __forceinline__ void weno7_pair(const double c[8], double& left, doub
     const double P2 = -c[2] + 7.0 * c[3] + 7.0 * c[4] - c[5];
     const double P3 = 3.0 * c[3] + 13.0 * c[4] - 5.0 * c[5] + c[6];
     const double R0 = -3.0 * c[7] + 13.0 * c[6] - 23.0 * c[5] + 25.0 * c[4];
+    constexpr double eps = 1e-6;
     double bL[4], bR[4];
-    bL[0] = w7_beta(c[0], c[1], c[2], c[3], I0);
-    bL[1] = w7_beta(c[1], c[2], c[3], c[4], I1);
-    bL[2] = w7_beta(c[2], c[3], c[4], c[5], I2);
-    bL[3] = w7_beta(c[3], c[4], c[5], c[6], I3);
-    bR[0] = w7_beta(c[7], c[6], c[5], c[4], I0);
-    bR[1] = w7_beta(c[6], c[5], c[4], c[3], I1);
-    bR[2] = w7_beta(c[5], c[4], c[3], c[2], I2);
-    bR[3] = w7_beta(c[4], c[3], c[2], c[1], I3);
+    bL[0] = w7_beta(c[0], c[1], c[2], c[3], I0, eps);
+    bL[1] = w7_beta(c[1], c[2], c[3], c[4], I1, eps);
+    bL[2] = w7_beta(c[2], c[3], c[4], c[5], I2, eps);
+    bL[3] = w7_beta(c[3], c[4], c[5], c[6], I3, eps);
+    bR[0] = w7_beta(c[7], c[6], c[5], c[4], I0, eps);
+    bR[1] = w7_beta(c[6], c[5], c[4], c[3], I1, eps);
+    bR[2] = w7_beta(c[5], c[4], c[3], c[2], I2, eps);
+    bR[3] = w7_beta(c[4], c[3], c[2], c[1], I3, eps);
     double wL[4], wR[4];
     weno7_side_weights(bL, wL);
     weno7_side_weights(bR, wR);
