@@ -144,14 +144,17 @@ __device__ __forceinline__ void carry_node(const GT& G, const StageArgs& A, long
     }
 }
 
-// Stage update of one node; S is the accumulated flux difference.  GT is
-// any descriptor with P, jac, gamma, active[3] and metric[3]; the step's dt
-// is read from *A.dt.  TIME fixes the time recipe (A.time) at compile time;
-// -1 reads it from A.
+// Stage update of one node from its current value qc and (SSPRK3 after
+// the first stage, RK4) its step-start value q0v; S is the accumulated flux
+// difference.  GT is any descriptor with P, jac, gamma, active[3] and
+// metric[3]; the step's dt is read from *A.dt.  TIME fixes the time recipe
+// (A.time) at compile time; -1 reads it from A.
 template <int TIME = -1, class GT>
-__device__ __forceinline__ void stage_update(const GT& G, const StageArgs& A, long long id,
-                                             const double* __restrict__ q, const double S[5],
-                                             unsigned& flags, double& dt_local, double qn[5]) {
+__device__ __forceinline__ void stage_update_values(const GT& G, const StageArgs& A,
+                                                    long long id, const double qc[5],
+                                                    const double q0v[5], const double S[5],
+                                                    unsigned& flags, double& dt_local,
+                                                    double qn[5]) {
     const int time = TIME >= 0 ? TIME : A.time;
     if (time == 3) {  // residual only (diagnostics)
 #pragma unroll
@@ -162,17 +165,16 @@ __device__ __forceinline__ void stage_update(const GT& G, const StageArgs& A, lo
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         const double R = G.jac * S[v];
-        const double qc = q[v * G.P + id];
         if (time == 1) {
-            const double euler = qc - dt * R;
-            qn[v] = (A.stage == 0 ? 0.0 : A.base * A.q0[v * G.P + id]) + A.incr * euler;
+            const double euler = qc[v] - dt * R;
+            qn[v] = (A.stage == 0 ? 0.0 : A.base * q0v[v]) + A.incr * euler;
         } else if (time == 0) {
-            qn[v] = qc - dt * R;
+            qn[v] = qc[v] - dt * R;
         } else if (A.stage < 3) {
             A.k[A.stage][v * G.P + id] = R;
-            qn[v] = A.q0[v * G.P + id] - A.incr * dt * R;
+            qn[v] = q0v[v] - A.incr * dt * R;
         } else {
-            qn[v] = A.q0[v * G.P + id]
+            qn[v] = q0v[v]
                   - dt / 6.0 * (A.k[0][v * G.P + id] + 2.0 * A.k[1][v * G.P + id]
                                 + 2.0 * A.k[2][v * G.P + id] + R);
         }
@@ -183,6 +185,26 @@ __device__ __forceinline__ void stage_update(const GT& G, const StageArgs& A, lo
     const double eint = qn[4] - ke;
     if (!(qn[0] > 0.0) || !(eint > 0.0) || !isfinite(eint)) flags |= kFlagState;
     if (A.reduce_dt) dt_local = local_dt(G, qn, A.cfl, flags);
+}
+
+// Whether a stage of recipe `time` reads the step-start state q0.
+__device__ __forceinline__ bool stage_reads_q0(int time, int stage) {
+    return (time == 1 && stage > 0) || time == 2;
+}
+
+// stage_update_values with qc and q0v read here.
+template <int TIME = -1, class GT>
+__device__ __forceinline__ void stage_update(const GT& G, const StageArgs& A, long long id,
+                                             const double* __restrict__ q, const double S[5],
+                                             unsigned& flags, double& dt_local, double qn[5]) {
+    const int time = TIME >= 0 ? TIME : A.time;
+    double qc[5], q0v[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        qc[v] = time == 3 ? 0.0 : q[v * G.P + id];
+        q0v[v] = stage_reads_q0(time, A.stage) ? A.q0[v * G.P + id] : 0.0;
+    }
+    stage_update_values<TIME>(G, A, id, qc, q0v, S, flags, dt_local, qn);
 }
 
 // Block-wide minimum of a positive double into *target (one atomic).

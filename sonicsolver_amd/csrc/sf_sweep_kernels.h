@@ -20,14 +20,32 @@ namespace sfamd {
 // ---------------------------------------------------------------------------
 constexpr int kXBlock = 64;
 
-// The final HLLC sweeps fit four waves per SIMD (128 VGPRs) without
-// spilling when asked to; the others stay at three (the final Rusanov sweeps
-// would spill at four).
+// The final sweep's operands for node id: its current value, its
+// accumulated flux difference and (where the recipe reads it) its
+// step-start value.
+template <int TIME>
+__device__ __forceinline__ void stage_operands(const GridDev& G, const StageArgs& A,
+                                               const double* __restrict__ q,
+                                               const double* __restrict__ S, long long id,
+                                               double (&pq)[5], double (&ps)[5],
+                                               double (&pq0)[5]) {
+    const bool with_q0 = stage_reads_q0(TIME, A.stage);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        pq[v] = q[v * G.P + id];
+        ps[v] = S[v * G.P + id];
+        pq0[v] = with_q0 ? A.q0[v * G.P + id] : 0.0;
+    }
+}
+
+// Three waves per SIMD: the registers that hold the final sweep's stage
+// operands through the face arithmetic pay more than a fourth wave would
+// (and four would spill).
 // TIME: the time recipe of a final sweep (A.time, fixed at compile time so
 // that the update carries no code for the other recipes); 0 otherwise.
 template <int SCHEME, int FLUX, int MODE, int TIME>
 __global__ __launch_bounds__(kXBlock)
-__attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) ? 4 : 3, 8))) void k_sweep_x(GridDev G, const double* __restrict__ q,
+__attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const double* __restrict__ q,
                                                      double* __restrict__ S, StageArgs A) {
     // face numbers fit 32 bits: the host rejects grids with 2^32 - 2^8
     // faces or more along x
@@ -51,6 +69,17 @@ __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) 
         const int a = face ? (int)(f - line * per_line) - 1 : 0;  // lower node of the face
         const int j = (int)(line % (unsigned)G.n[1]), k = (int)(line / (unsigned)G.n[1]);
         double f5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        // The final sweep's stage operands for the node this lane updates
+        // (the lower node of its face): loaded before the face arithmetic,
+        // so that their latency hides behind it -- except for WENO7, whose
+        // eight-node stencil leaves no registers to hold them at three waves
+        // per SIMD; it loads them after the face.
+        const bool upd = t >= 1 && face && a >= 0;
+        double pq[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        double pq0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        constexpr bool kEarly = SCHEME != 3;
+        if (MODE == kFinal && kEarly && upd)
+            stage_operands<TIME>(G, A, q, S, node_index(G, a, j, k), pq, ps, pq0);
         if (face) {
             constexpr int NS = Stencil<SCHEME>::ns;
             const long long base = node_index(G, a, j, k);
@@ -67,19 +96,28 @@ __attribute__((amdgpu_waves_per_eu((MODE == kFinal && FLUX == 1 && SCHEME <= 1) 
         double lower[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) lower[v] = __shfl_up(f5[v], 1, kXBlock);
-        if (t >= 1 && face && a >= 0) {
+        if (upd) {
+            if (MODE == kFinal && !kEarly)
+                stage_operands<TIME>(G, A, q, S, node_index(G, a, j, k), pq, ps, pq0);
             const long long id = node_index(G, a, j, k);
             double s5[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 const double d = f5[v] - lower[v];
-                s5[v] = (MODE == kCopyS) ? d : S[v * G.P + id] + d;
+                s5[v] = (MODE == kCopyS) ? d : ((MODE == kFinal) ? ps[v] : S[v * G.P + id]) + d;
             }
             if (MODE == kFinal) {
                 double dl = INFINITY;
                 double qn[5];
-                if (solved_node(G, a, j, k)) stage_update<TIME>(G, A, id, q, s5, flags, dl, qn);
-                else carry_node<TIME>(G, A, id, q, qn);
+                if (solved_node(G, a, j, k)) {
+                    stage_update_values<TIME>(G, A, id, pq, pq0, s5, flags, dl, qn);
+                } else if (TIME != 3) {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) {
+                        qn[v] = pq[v];   // carry_node
+                        A.qout[v * G.P + id] = qn[v];
+                    }
+                }
                 if (TIME != 3 && G.images) write_images(G, A.qout, a, j, k, id, qn);
                 dt_local = fmin(dt_local, dl);
             } else {
