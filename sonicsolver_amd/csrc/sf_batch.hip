@@ -126,6 +126,20 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
         const bool face = f < per_line * it.n1;
         const int j = face ? f / per_line : 0;
         const int a = f - j * per_line - 1;  // lower node of the face
+        // the stage operands of the node this lane updates, loaded before
+        // the face arithmetic so that their latency hides behind it
+        const bool upd = t >= 1 && face && a >= 0;
+        double pq[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        double pq0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (upd) {
+            const long long idu = item_index(it, a, j);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                pq[v] = q[v * B.P + idu];
+                if (it.n1 > 1) ps[v] = S[v * B.P + idu];
+                if (st.stage > 0) pq0[v] = st.q0[v * B.P + idu];
+            }
+        }
         if (face) {
             const long long base = item_index(it, a, j);
             double stc[6][5];
@@ -143,13 +157,13 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
         }
         __syncthreads();
         double dl = INFINITY;
-        if (t >= 1 && face && a >= 0) {
+        if (upd) {
             const long long id = item_index(it, a, j);
             double s5[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 const double d = sh[v][t] - sh[v][t - 1];
-                s5[v] = (it.n1 > 1) ? S[v * B.P + id] + d : d;
+                s5[v] = (it.n1 > 1) ? ps[v] + d : d;
             }
             StageArgs A{};
             A.q0 = st.q0;
@@ -164,8 +178,15 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
             A.reduce_dt = st.reduce_dt;
             const ItemView G = item_view(B, it);
             double qn[5];
-            if (item_solved(it, a, j)) stage_update(G, A, id, q, s5, flags, dl, qn);
-            else carry_node(G, A, id, q, qn);
+            if (item_solved(it, a, j)) {
+                stage_update_values<SF_TIME_SSPRK3>(G, A, id, pq, pq0, s5, flags, dl, qn);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    qn[v] = pq[v];   // carry_node
+                    A.qout[v * B.P + id] = qn[v];
+                }
+            }
         }
         if (st.reduce_dt) block_min_atomic(dl, &B.dtmin[T.x], red);
         if (flags) {
