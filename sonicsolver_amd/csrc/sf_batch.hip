@@ -110,7 +110,21 @@ __device__ __forceinline__ bool item_solved(const BatchItemDev& it, int i, int j
 // with their left neighbour's face through LDS and update that node.  Short
 // lines therefore share a block and long ones span several.  Block b0 of nb
 // takes tiles t0+b0, t0+b0+nb, ... (persistent).
-template <int BS, int SCHEME, int FLUX>
+__device__ __forceinline__ void batch_operands(const BatchDev& B, const BatchItemDev& it,
+                                               const BatchStage& st,
+                                               const double* __restrict__ q,
+                                               const double* __restrict__ S, long long id,
+                                               double (&pq)[5], double (&ps)[5],
+                                               double (&pq0)[5]) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        pq[v] = q[v * B.P + id];
+        ps[v] = it.n1 > 1 ? S[v * B.P + id] : 0.0;
+        pq0[v] = st.stage > 0 ? st.q0[v * B.P + id] : 0.0;
+    }
+}
+
+template <int BS, int SCHEME, int FLUX, bool EARLY>
 __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restrict__ q,
                                         const double* __restrict__ S, const BatchStage& st,
                                         int t0, int t1, int b0, int nb, double (*sh)[BS],
@@ -126,20 +140,16 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
         const bool face = f < per_line * it.n1;
         const int j = face ? f / per_line : 0;
         const int a = f - j * per_line - 1;  // lower node of the face
-        // the stage operands of the node this lane updates, loaded before
-        // the face arithmetic so that their latency hides behind it
+        // The stage operands of the node this lane updates.  With EARLY they
+        // are loaded before the face arithmetic, so that their latency hides
+        // behind it.  For WENO5 that costs the registers of a fourth wave
+        // per SIMD, which pays where 2D problems read S and not for 1D-only
+        // batches; TENO5 runs at three waves either way and always loads
+        // early.
         const bool upd = t >= 1 && face && a >= 0;
         double pq[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         double pq0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (upd) {
-            const long long idu = item_index(it, a, j);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                pq[v] = q[v * B.P + idu];
-                if (it.n1 > 1) ps[v] = S[v * B.P + idu];
-                if (st.stage > 0) pq0[v] = st.q0[v * B.P + idu];
-            }
-        }
+        if (EARLY && upd) batch_operands(B, it, st, q, S, item_index(it, a, j), pq, ps, pq0);
         if (face) {
             const long long base = item_index(it, a, j);
             double stc[6][5];
@@ -159,6 +169,7 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
         double dl = INFINITY;
         if (upd) {
             const long long id = item_index(it, a, j);
+            if (!EARLY) batch_operands(B, it, st, q, S, id, pq, ps, pq0);
             double s5[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
@@ -216,23 +227,23 @@ __global__ __launch_bounds__(BS) void k_batch_x_fused(BatchDev B, const double* 
     while (v < 3 && bid >= sp.block[v + 1]) ++v;
     const int b0 = bid - sp.block[v], nb = sp.block[v + 1] - sp.block[v];
     switch (v) {
-        case 0: x_tiles<BS, 0, 0>(B, q, S, st, sp.tile[0], sp.tile[1], b0, nb, sh, red); break;
-        case 1: x_tiles<BS, 0, 1>(B, q, S, st, sp.tile[1], sp.tile[2], b0, nb, sh, red); break;
-        case 2: x_tiles<BS, 1, 0>(B, q, S, st, sp.tile[2], sp.tile[3], b0, nb, sh, red); break;
-        default: x_tiles<BS, 1, 1>(B, q, S, st, sp.tile[3], sp.tile[4], b0, nb, sh, red); break;
+        case 0: x_tiles<BS, 0, 0, true>(B, q, S, st, sp.tile[0], sp.tile[1], b0, nb, sh, red); break;
+        case 1: x_tiles<BS, 0, 1, true>(B, q, S, st, sp.tile[1], sp.tile[2], b0, nb, sh, red); break;
+        case 2: x_tiles<BS, 1, 0, true>(B, q, S, st, sp.tile[2], sp.tile[3], b0, nb, sh, red); break;
+        default: x_tiles<BS, 1, 1, true>(B, q, S, st, sp.tile[3], sp.tile[4], b0, nb, sh, red); break;
     }
 }
 
 // One launch per variant (fewer registers than the fused form).  Three waves
 // per SIMD fit without spilling except for TENO5 + HLLC, which keeps two.
-template <int BS, int SCHEME, int FLUX>
+template <int BS, int SCHEME, int FLUX, bool EARLY>
 __global__ __launch_bounds__(BS)
 __attribute__((amdgpu_waves_per_eu((SCHEME == 1 && FLUX == 1) ? 2 : 3, 8))) void k_batch_x(BatchDev B, const double* __restrict__ q,
                                                 const double* __restrict__ S, BatchStage st,
                                                 int t0, int t1) {
     __shared__ double sh[5][BS];
     __shared__ unsigned long long red[BS / 64];
-    x_tiles<BS, SCHEME, FLUX>(B, q, S, st, t0, t1, blockIdx.x, gridDim.x, sh, red);
+    x_tiles<BS, SCHEME, FLUX, EARLY>(B, q, S, st, t0, t1, blockIdx.x, gridDim.x, sh, red);
 }
 
 // y faces of the 2D problems over tiles [t0, t1) of one variant.  A
@@ -673,7 +684,11 @@ void launch_x(sf_batch* b, const double* q, BatchStage st) {
         const int t0 = b->xr[v], t1 = b->xr[v + 1];
         if (t1 == t0) continue;
         const unsigned g = persistent_blocks(b, t1 - t0);
-#define SF_XV(BS, SC, FL) k_batch_x<BS, SC, FL><<<g, BS, 0, b->stream>>>(B, q, b->S, st, t0, t1)
+#define SF_XV(BS, SC, FL)                                                                   \
+    do {                                                                                   \
+        if (b->any2d || SC == 1) k_batch_x<BS, SC, FL, true><<<g, BS, 0, b->stream>>>(B, q, b->S, st, t0, t1); \
+        else k_batch_x<BS, SC, FL, false><<<g, BS, 0, b->stream>>>(B, q, b->S, st, t0, t1);  \
+    } while (0)
 #define SF_XBS(SC, FL)                        \
     switch (b->xbs) {                         \
         case 64: SF_XV(64, SC, FL); break;    \
