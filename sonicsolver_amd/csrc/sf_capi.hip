@@ -197,13 +197,15 @@ int run_sweeps(sf_grid* g, const double* q, const StageArgs& A) {
     for (int t = 0; t < na; ++t) {
         const int mode = (t == na - 1) ? kFinal : (t == 0) ? kCopyS : kAccumS;
         const int ev = prof_start(g, SF_KERNEL_SWEEP_X + axes[t]);
+        int rc;
         switch (g->desc.scheme) {
-        case SF_SCHEME_TENO5: launch_sweep<1>(c, g->desc.flux, axes[t], mode); break;
-        case SF_SCHEME_WENO3: launch_sweep<2>(c, g->desc.flux, axes[t], mode); break;
-        case SF_SCHEME_WENO7: launch_sweep<3>(c, g->desc.flux, axes[t], mode); break;
-        default: launch_sweep<0>(c, g->desc.flux, axes[t], mode); break;
+        case SF_SCHEME_TENO5: rc = launch_sweep<1>(c, g->desc.flux, axes[t], mode); break;
+        case SF_SCHEME_WENO3: rc = launch_sweep<2>(c, g->desc.flux, axes[t], mode); break;
+        case SF_SCHEME_WENO7: rc = launch_sweep<3>(c, g->desc.flux, axes[t], mode); break;
+        default: rc = launch_sweep<0>(c, g->desc.flux, axes[t], mode); break;
         }
         prof_stop(g, ev);
+        if (rc) return fail(SF_E_STATE, "run_sweeps: no kernel for this sweep axis and mode");
     }
     SF_HIP(hipGetLastError());
     return 0;

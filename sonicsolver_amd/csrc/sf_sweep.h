@@ -20,62 +20,71 @@ struct SweepCall {
     int seg;           // nodes per march segment (y and z sweeps)
 };
 
+// The x sweep is always the last sweep of a stage (x is always active and
+// the sweeps run z, y, x), so it is compiled in its final mode only, and the
+// y / z marches in their S-writing modes only.  Returns -1 for a combination
+// that is not compiled.
 template <int SCHEME, int FLUX, int MODE>
-void launch_sweep_fm(const SweepCall& c, int axis) {
+int launch_sweep_fm(const SweepCall& c, int axis) {
     const GridDev& G = c.G;
     if (axis == 0) {
-        // up to 16384 blocks (a multiple of 8: the kernel deals tiles to
-        // XCDs), each looping over tiles of kXBlock faces
-        const long long faces = (long long)(G.n[0] + 1) * G.n[1] * G.n[2];
-        const long long tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
-        const unsigned bx = (unsigned)((std::min<long long>(tiles, 16384) + 7) & ~7LL);
-        if (MODE != kFinal) {
-            k_sweep_x<SCHEME, FLUX, MODE, 0><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A);
-            return;
+        if constexpr (MODE != kFinal) {
+            return -1;
+        } else {
+            // up to 16384 blocks (a multiple of 8: the kernel deals tiles to
+            // XCDs), each looping over tiles of kXBlock faces
+            const long long faces = (long long)(G.n[0] + 1) * G.n[1] * G.n[2];
+            const long long tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
+            const unsigned bx = (unsigned)((std::min<long long>(tiles, 16384) + 7) & ~7LL);
+            switch (c.A.time) {
+            case 0: k_sweep_x<SCHEME, FLUX, MODE, 0><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
+            case 1: k_sweep_x<SCHEME, FLUX, MODE, 1><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
+            case 2: k_sweep_x<SCHEME, FLUX, MODE, 2><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
+            default: k_sweep_x<SCHEME, FLUX, MODE, 3><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
+            }
+            return 0;
         }
-        switch (c.A.time) {
-        case 0: k_sweep_x<SCHEME, FLUX, MODE, 0><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
-        case 1: k_sweep_x<SCHEME, FLUX, MODE, 1><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
-        case 2: k_sweep_x<SCHEME, FLUX, MODE, 2><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
-        default: k_sweep_x<SCHEME, FLUX, MODE, 3><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
-        }
-        return;
     }
-    const unsigned other = (unsigned)(axis == 1 ? G.n[2] : G.n[1]);
-    const dim3 grid((unsigned)((G.n[0] + kMarchBlock - 1) / kMarchBlock), other,
-                    (unsigned)((G.n[axis] + c.seg - 1) / c.seg));
-    if (axis == 1)
-        k_sweep_march<1, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, c.stream>>>(G, c.q, c.S, c.A,
-                                                                                c.seg);
-    else
-        k_sweep_march<2, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, c.stream>>>(G, c.q, c.S, c.A,
-                                                                                c.seg);
+    if constexpr (MODE == kFinal) {
+        return -1;
+    } else {
+        const unsigned other = (unsigned)(axis == 1 ? G.n[2] : G.n[1]);
+        const dim3 grid((unsigned)((G.n[0] + kMarchBlock - 1) / kMarchBlock), other,
+                        (unsigned)((G.n[axis] + c.seg - 1) / c.seg));
+        if (axis == 1)
+            k_sweep_march<1, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, c.stream>>>(
+                G, c.q, c.S, c.A, c.seg);
+        else
+            k_sweep_march<2, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, c.stream>>>(
+                G, c.q, c.S, c.A, c.seg);
+        return 0;
+    }
 }
 
 template <int SCHEME, int FLUX>
-void launch_sweep_f(const SweepCall& c, int axis, int mode) {
-    if (mode == kCopyS) launch_sweep_fm<SCHEME, FLUX, kCopyS>(c, axis);
-    else if (mode == kAccumS) launch_sweep_fm<SCHEME, FLUX, kAccumS>(c, axis);
-    else launch_sweep_fm<SCHEME, FLUX, kFinal>(c, axis);
+int launch_sweep_f(const SweepCall& c, int axis, int mode) {
+    if (mode == kCopyS) return launch_sweep_fm<SCHEME, FLUX, kCopyS>(c, axis);
+    if (mode == kAccumS) return launch_sweep_fm<SCHEME, FLUX, kAccumS>(c, axis);
+    return launch_sweep_fm<SCHEME, FLUX, kFinal>(c, axis);
 }
 
 // One sweep of scheme SCHEME with face flux `flux` (SF_FLUX_*) along
 // `axis` in stage mode `mode` (kCopyS, kAccumS, kFinal).
 template <int SCHEME>
-void launch_sweep(const SweepCall& c, int flux, int axis, int mode) {
+int launch_sweep(const SweepCall& c, int flux, int axis, int mode) {
     switch (flux) {
-    case 0: launch_sweep_f<SCHEME, 0>(c, axis, mode); break;
-    case 1: launch_sweep_f<SCHEME, 1>(c, axis, mode); break;
-    case 2: launch_sweep_f<SCHEME, 2>(c, axis, mode); break;
-    case 3: launch_sweep_f<SCHEME, 3>(c, axis, mode); break;
-    case 4: launch_sweep_f<SCHEME, 4>(c, axis, mode); break;
-    default: launch_sweep_f<SCHEME, 5>(c, axis, mode); break;
+    case 0: return launch_sweep_f<SCHEME, 0>(c, axis, mode);
+    case 1: return launch_sweep_f<SCHEME, 1>(c, axis, mode);
+    case 2: return launch_sweep_f<SCHEME, 2>(c, axis, mode);
+    case 3: return launch_sweep_f<SCHEME, 3>(c, axis, mode);
+    case 4: return launch_sweep_f<SCHEME, 4>(c, axis, mode);
+    default: return launch_sweep_f<SCHEME, 5>(c, axis, mode);
     }
 }
 
-extern template void launch_sweep<0>(const SweepCall&, int, int, int);
-extern template void launch_sweep<1>(const SweepCall&, int, int, int);
-extern template void launch_sweep<2>(const SweepCall&, int, int, int);
-extern template void launch_sweep<3>(const SweepCall&, int, int, int);
+extern template int launch_sweep<0>(const SweepCall&, int, int, int);
+extern template int launch_sweep<1>(const SweepCall&, int, int, int);
+extern template int launch_sweep<2>(const SweepCall&, int, int, int);
+extern template int launch_sweep<3>(const SweepCall&, int, int, int);
 
 }  // namespace sfamd

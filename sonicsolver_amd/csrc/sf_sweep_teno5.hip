@@ -3,5 +3,5 @@
 #include "sf_sweep.h"
 
 namespace sfamd {
-template void launch_sweep<1>(const SweepCall&, int, int, int);
+template int launch_sweep<1>(const SweepCall&, int, int, int);
 }  // namespace sfamd
