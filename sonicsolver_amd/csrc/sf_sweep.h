@@ -37,10 +37,10 @@ int launch_sweep_fm(const SweepCall& c, int axis) {
             const long long tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
             const unsigned bx = (unsigned)((std::min<long long>(tiles, 16384) + 7) & ~7LL);
             switch (c.A.time) {
-            case 0: k_sweep_x<SCHEME, FLUX, MODE, 0><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
-            case 1: k_sweep_x<SCHEME, FLUX, MODE, 1><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
-            case 2: k_sweep_x<SCHEME, FLUX, MODE, 2><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
-            default: k_sweep_x<SCHEME, FLUX, MODE, 3><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
+            case 0: k_sweep_x<SCHEME, FLUX, 0><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
+            case 1: k_sweep_x<SCHEME, FLUX, 1><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
+            case 2: k_sweep_x<SCHEME, FLUX, 2><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
+            default: k_sweep_x<SCHEME, FLUX, 3><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
             }
             return 0;
         }

@@ -20,9 +20,9 @@ namespace sfamd {
 // ---------------------------------------------------------------------------
 constexpr int kXBlock = 64;
 
-// The final sweep's operands for node id: its current value, its
-// accumulated flux difference and (where the recipe reads it) its
-// step-start value.
+// The final sweep's operands for node id: its current value, the flux
+// difference the y and z sweeps accumulated (none on a 1D grid) and (where
+// the recipe reads it) its step-start value.
 template <int TIME>
 __device__ __forceinline__ void stage_operands(const GridDev& G, const StageArgs& A,
                                                const double* __restrict__ q,
@@ -30,10 +30,11 @@ __device__ __forceinline__ void stage_operands(const GridDev& G, const StageArgs
                                                double (&pq)[5], double (&ps)[5],
                                                double (&pq0)[5]) {
     const bool with_q0 = stage_reads_q0(TIME, A.stage);
+    const bool with_s = G.active[1];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         pq[v] = q[v * G.P + id];
-        ps[v] = S[v * G.P + id];
+        ps[v] = with_s ? S[v * G.P + id] : 0.0;
         pq0[v] = with_q0 ? A.q0[v * G.P + id] : 0.0;
     }
 }
@@ -41,12 +42,14 @@ __device__ __forceinline__ void stage_operands(const GridDev& G, const StageArgs
 // Three waves per SIMD: the registers that hold the final sweep's stage
 // operands through the face arithmetic pay more than a fourth wave would
 // (and four would spill).
-// TIME: the time recipe of a final sweep (A.time, fixed at compile time so
-// that the update carries no code for the other recipes); 0 otherwise.
-template <int SCHEME, int FLUX, int MODE, int TIME>
+// The x sweep is always a stage's last sweep (x is always active and the
+// sweeps run z, y, x), so it always applies the stage update.  TIME: the
+// time recipe (A.time, fixed at compile time so that the update carries no
+// code for the other recipes).
+template <int SCHEME, int FLUX, int TIME>
 __global__ __launch_bounds__(kXBlock)
 __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const double* __restrict__ q,
-                                                     double* __restrict__ S, StageArgs A) {
+                                                     const double* __restrict__ S, StageArgs A) {
     // face numbers fit 32 bits: the host rejects grids with 2^32 - 2^8
     // faces or more along x
     const int t = threadIdx.x;
@@ -78,8 +81,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
         double pq[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         double pq0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         constexpr bool kEarly = SCHEME != 3;
-        if (MODE == kFinal && kEarly && upd)
-            stage_operands<TIME>(G, A, q, S, node_index(G, a, j, k), pq, ps, pq0);
+        if (kEarly && upd) stage_operands<TIME>(G, A, q, S, node_index(G, a, j, k), pq, ps, pq0);
         if (face) {
             constexpr int NS = Stencil<SCHEME>::ns;
             const long long base = node_index(G, a, j, k);
@@ -97,37 +99,31 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
 #pragma unroll
         for (int v = 0; v < 5; ++v) lower[v] = __shfl_up(f5[v], 1, kXBlock);
         if (upd) {
-            if (MODE == kFinal && !kEarly)
-                stage_operands<TIME>(G, A, q, S, node_index(G, a, j, k), pq, ps, pq0);
+            if (!kEarly) stage_operands<TIME>(G, A, q, S, node_index(G, a, j, k), pq, ps, pq0);
             const long long id = node_index(G, a, j, k);
             double s5[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 const double d = f5[v] - lower[v];
-                s5[v] = (MODE == kCopyS) ? d : ((MODE == kFinal) ? ps[v] : S[v * G.P + id]) + d;
+                s5[v] = ps[v] + d;
             }
-            if (MODE == kFinal) {
-                double dl = INFINITY;
-                double qn[5];
-                if (solved_node(G, a, j, k)) {
-                    stage_update_values<TIME>(G, A, id, pq, pq0, s5, flags, dl, qn);
-                } else if (TIME != 3) {
+            double dl = INFINITY;
+            double qn[5];
+            if (solved_node(G, a, j, k)) {
+                stage_update_values<TIME>(G, A, id, pq, pq0, s5, flags, dl, qn);
+            } else if (TIME != 3) {
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) {
-                        qn[v] = pq[v];   // carry_node
-                        A.qout[v * G.P + id] = qn[v];
-                    }
+                for (int v = 0; v < 5; ++v) {
+                    qn[v] = pq[v];   // carry_node
+                    A.qout[v * G.P + id] = qn[v];
                 }
-                if (TIME != 3 && G.images) write_images(G, A.qout, a, j, k, id, qn);
-                dt_local = fmin(dt_local, dl);
-            } else {
-#pragma unroll
-                for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
             }
+            if (TIME != 3 && G.images) write_images(G, A.qout, a, j, k, id, qn);
+            dt_local = fmin(dt_local, dl);
         }
     }
     raise_flags(flags, A.flags);
-    if (MODE == kFinal && A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);
+    if (A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);
 }
 
 // ---------------------------------------------------------------------------
@@ -154,8 +150,8 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
     const int o = blockIdx.y;
     const int a0 = blockIdx.z * seg;
     const int a1 = min(a0 + seg, G.n[AXIS]);
+    static_assert(MODE == kCopyS || MODE == kAccumS, "the x sweep is a stage's final sweep");
     unsigned flags = 0;
-    double dt_local = INFINITY;
     const double lw = (FLUX == 5) ? *A.dt / G.d[AXIS] : 0.0;   // Lax-Wendroff dt / spacing
     if (i < G.n[0]) {
         const int j0 = (AXIS == 1) ? a0 : o;
@@ -209,23 +205,11 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
 #pragma unroll
             for (int v = 0; v < 5; ++v)
                 if (MODE != kCopyS) s5[v] = S[v * G.P + id] + s5[v];
-            if (MODE == kFinal) {
-                const int jj = (AXIS == 1) ? a : o;
-                const int kk = (AXIS == 2) ? a : o;
-                double dl = INFINITY;
-                double qn[5];
-                if (solved_node(G, i, jj, kk)) stage_update(G, A, id, q, s5, flags, dl, qn);
-                else carry_node(G, A, id, q, qn);
-                if (G.images && A.time != 3) write_images(G, A.qout, i, jj, kk, id, qn);
-                dt_local = fmin(dt_local, dl);
-            } else {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
-            }
+            for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
         }
     }
     raise_flags(flags, A.flags);
-    if (MODE == kFinal && A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);
 }
 
 }  // namespace sfamd
