@@ -17,7 +17,7 @@ struct SweepCall {
     double* S;         // flux-difference accumulator
     StageArgs A;
     hipStream_t stream;
-    int seg;           // nodes per march segment (y and z sweeps)
+    int seg;           // base nodes per march segment (y and z sweeps)
 };
 
 // The x sweep is always the last sweep of a stage (x is always active and
@@ -49,14 +49,20 @@ int launch_sweep_fm(const SweepCall& c, int axis) {
         return -1;
     } else {
         const unsigned other = (unsigned)(axis == 1 ? G.n[2] : G.n[1]);
-        const dim3 grid((unsigned)((G.n[0] + kMarchBlock - 1) / kMarchBlock), other,
-                        (unsigned)((G.n[axis] + c.seg - 1) / c.seg));
+        const unsigned cols = (unsigned)((G.n[0] + kMarchBlock - 1) / kMarchBlock);
+        // Each segment re-reads the stencil window and evaluates one extra
+        // face at its start, so segments twice the base length are used
+        // wherever that still leaves 8192 blocks or more to fill the chip
+        // (512^3: 64 nodes).
+        int seg = c.seg;
+        if ((long long)cols * other * ((G.n[axis] + 2 * seg - 1) / (2 * seg)) >= 8192) seg *= 2;
+        const dim3 grid(cols, other, (unsigned)((G.n[axis] + seg - 1) / seg));
         if (axis == 1)
             k_sweep_march<1, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, c.stream>>>(
-                G, c.q, c.S, c.A, c.seg);
+                G, c.q, c.S, c.A, seg);
         else
             k_sweep_march<2, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, c.stream>>>(
-                G, c.q, c.S, c.A, c.seg);
+                G, c.q, c.S, c.A, seg);
         return 0;
     }
 }
