@@ -434,9 +434,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU only; the explicit hot path is measured "
-              "on a single MI355X (multi-GPU sharding is a design note, not a measured "
-              "configuration). Re-run with --gpus 1.", file=sys.stderr)
+        print("bench.py: sonicsolver_amd measures the explicit hot path on a single MI355X "
+              "only (multi-GPU sharding is a design note, not a measured configuration). "
+              "Re-run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from sonicsolver_amd import Grid, make_desc, workloads as wl, flops

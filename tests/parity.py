@@ -1,7 +1,7 @@
 """Comparison helpers shared by the parity tests."""
 import numpy as np
 
-# relative L1 and Linf error on conserved variables.
+# Parity tolerance: relative L1 and Linf error on the conserved variables.
 REL_TOL = 1e-11
 
 
