@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
     const int o = blockIdx.y;
     const int a0 = blockIdx.z * seg;
     const int a1 = min(a0 + seg, G.n[AXIS]);
-    static_assert(MODE == kCopyS || MODE == kAccumS, "the x sweep is a stage's final sweep");
+    static_assert(MODE == kCopyS || MODE == kAccumS, "a march writes or accumulates S; the x sweep applies the update");
     unsigned flags = 0;
     const double lw = (FLUX == 5) ? *A.dt / G.d[AXIS] : 0.0;   // Lax-Wendroff dt / spacing
     if (i < G.n[0]) {
