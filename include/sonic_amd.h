@@ -146,7 +146,13 @@ int sf_stage_count(const sf_grid *grid, int *count);
 /* nsteps physical steps with dt = min(CFL dt, max_dt) each, all queued on
  * the device without host synchronisation between steps.  If dt_seq is not
  * NULL it receives the nsteps dt values.  Checks the device error flags at
- * the end and returns SF_E_NUMERIC if any was raised. */
+ * the end and returns SF_E_NUMERIC if any was raised, as sf_fault_check
+ * does: the kernels queued after the fault did nothing, the grid holds the
+ * state at the start of the faulting step and its clock counts the steps
+ * before it (dt_seq entries from the faulting step on are not meaningful).
+ * The reference stops at the first non-physical stage
+ * (validateEquationState, SF_equationState.h:25-57; TENO5,
+ * SF_TENO5.cpp:79-89). */
 int sf_run(sf_grid *grid, int nsteps, double cfl, double max_dt,
            double *dt_seq);
 /* Physical time and step count of the grid's clock. */
@@ -157,8 +163,26 @@ int sf_clock(sf_grid *grid, double *time, long long *steps);
  * boundary preparation, on the unique nodes (0 on solver-boundary nodes);
  * Math::localSpatialResidual, SF_residualAssembly.h:12-32. */
 int sf_residual(sf_grid *grid, double *soa_out);
-/* Device error flags (SF_FLAG_*); clear != 0 resets them. */
+/* Device error flags (SF_FLAG_*); clear != 0 resets them (and the record
+ * of where they were raised).  sf_state_upload* also resets them. */
 int sf_error_flags(sf_grid *grid, uint32_t *flags, int clear);
+/* Waits for the grid's queued work and checks the device error flags, as
+ * the reference's fail-fast checks do after every stage
+ * (SF_equationState.h:25-57, SF_TENO5.cpp:79-89, SF_rusanovEOS.cpp:141-146).
+ * Once a flag is raised every later stage, fill and clock kernel of the
+ * grid returns at entry, so the buffer holding the state at the start of
+ * the faulting step is untouched; this call makes it the current state
+ * again.  Returns 0 with *flags = 0 when no flag is raised, else
+ * SF_E_NUMERIC with the flags, the 1-based number of the faulting step on
+ * the grid's clock and its phase (0..3 = stage 1..4, 6 = ghost fill at
+ * the start or commit of the step, 7 = the CFL scan before it).  Any
+ * output may be NULL. */
+int sf_fault_check(sf_grid *grid, uint32_t *flags, long long *step, int *phase);
+/* Nodes per segment of the y / z marching sweeps (1..4096), or 0 for the
+ * automatic choice (64 where that leaves at least 8192 one-wave blocks,
+ * else 32 halved down to 8 until there are 4096).  No reference
+ * counterpart; results do not depend on it. */
+int sf_grid_set_march_segment(sf_grid *grid, int nodes);
 /* Wait for the grid's queued work. */
 int sf_synchronize(sf_grid *grid);
 /* The HIP stream (hipStream_t) the grid queues its work on. */
@@ -245,8 +269,10 @@ int sf_batch_set_occupancy(sf_batch *batch, int blocks_per_cu);
  * depend on it. */
 int sf_batch_set_schedule(sf_batch *batch, int fused);
 /* Per-problem device fault flags (SF_FLAG_*), item_capacity entries (0 for
- * free slots).  A problem that raises a flag is frozen at that step; the
- * others keep advancing, and sf_batch_step returns SF_E_NUMERIC. */
+ * free slots).  A problem that raises a flag during a step finishes that
+ * step's stages on its own nodes, is not counted as having taken the step
+ * and is frozen from the next step on; the others keep advancing, and
+ * sf_batch_step returns SF_E_NUMERIC. */
 int sf_batch_faults(sf_batch *batch, uint32_t *flags);
 
 /* Launch shape chosen by sf_batch_autotune. */

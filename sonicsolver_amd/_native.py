@@ -79,6 +79,9 @@ def _declare(lib):
         "sf_clock": [_VP, _P, ctypes.POINTER(ctypes.c_longlong)],
         "sf_residual": [_VP, _P],
         "sf_error_flags": [_VP, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int],
+        "sf_fault_check": [_VP, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_longlong),
+                           ctypes.POINTER(ctypes.c_int)],
+        "sf_grid_set_march_segment": [_VP, ctypes.c_int],
         "sf_synchronize": [_VP],
         "sf_profile_enable": [_VP, ctypes.c_int],
         "sf_profile_read": [_VP, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_longlong)],
@@ -223,6 +226,21 @@ class Grid:
         f = ctypes.c_uint32(0)
         _check(load().sf_error_flags(self._h, ctypes.byref(f), int(clear)))
         return f.value
+
+    def fault(self):
+        """(flags, faulting step on the grid's clock, phase) after waiting for
+        the queued work; (0, 0, -1) without a fault.  With a fault the grid
+        is left holding the state at the start of the faulting step."""
+        f = ctypes.c_uint32(0)
+        st = ctypes.c_longlong(0)
+        ph = ctypes.c_int(-1)
+        rc = load().sf_fault_check(self._h, ctypes.byref(f), ctypes.byref(st), ctypes.byref(ph))
+        if rc not in (0, -3):
+            _check(rc)
+        return f.value, st.value, ph.value
+
+    def set_march_segment(self, nodes: int):
+        _check(load().sf_grid_set_march_segment(self._h, int(nodes)))
 
     def profile(self, enable: bool = True):
         _check(load().sf_profile_enable(self._h, int(enable)))

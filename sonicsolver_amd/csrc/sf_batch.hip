@@ -56,6 +56,7 @@ struct BatchDev {
     long long* steps;
     const long long* target;  // a problem is live while steps < target ...
     unsigned* flags;          // ... and raised no fault flags (per problem)
+    int* live;                // liveness fixed for the current step
 };
 
 struct BatchStage {
@@ -90,6 +91,16 @@ __device__ __forceinline__ ItemView item_view(const BatchDev& B, const BatchItem
 
 __device__ __forceinline__ bool item_live(const BatchDev& B, int slot) {
     return B.steps[slot] < B.target[slot] && B.flags[slot] == 0;
+}
+
+// Liveness for the kernels of one step: fixed by k_batch_finalize at the
+// start of the step, so every wave of every kernel of the step takes the
+// same decision for a problem, even while another block raises its flags
+// (a multi-wave x tile must agree, since its waves meet at barriers).  A
+// problem that faults during a step finishes that step's stages on its own
+// nodes, is not committed, and is frozen from the next step on.
+__device__ __forceinline__ bool step_live(const BatchDev& B, int slot) {
+    return B.live[slot] != 0;
 }
 
 __device__ __forceinline__ long long item_index(const BatchItemDev& it, int i, int j) {
@@ -133,7 +144,7 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
     unsigned flags = 0;
     for (int tile = t0 + b0; tile < t1; tile += nb) {
         const int4 T = B.xtiles[tile];
-        if (!item_live(B, T.x)) continue;  // uniform over the block
+        if (!step_live(B, T.x)) continue;  // uniform over the block
         const BatchItemDev it = B.items[T.x];
         const int per_line = it.n0 + 1;
         const int f = T.y + t;
@@ -260,7 +271,7 @@ __device__ __forceinline__ void y_tiles(const BatchDev& B, const double* __restr
     unsigned flags = 0;
     for (int tile = t0 + b0; tile < t1; tile += nb) {
         const int4 T = B.ytiles[tile];
-        if (!item_live(B, T.x)) continue;
+        if (!step_live(B, T.x)) continue;
         const BatchItemDev it = B.items[T.x];
         const int k = T.y + lane;
         const int nseg = (it.n1 + B.yseg - 1) / B.yseg;
@@ -355,7 +366,7 @@ __attribute__((amdgpu_waves_per_eu((SCHEME == 0) ? 4 : 3, 8))) void k_batch_y(Ba
 template <int AXIS, int PHASE>
 __global__ __launch_bounds__(256) void k_batch_fill(BatchDev B, double* __restrict__ q) {
     const int slot = B.active[blockIdx.x];
-    if (!item_live(B, slot)) return;
+    if (!step_live(B, slot)) return;
     const BatchItemDev it = B.items[slot];
     if (AXIS == 1 && it.n1 <= 1) return;
     const int nother = AXIS == 0 ? it.n1 : it.n0;
@@ -419,7 +430,9 @@ __global__ void k_batch_finalize(BatchDev B) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= B.nactive) return;
     const int slot = B.active[k];
-    if (!item_live(B, slot)) return;
+    const bool live = item_live(B, slot);
+    B.live[slot] = live ? 1 : 0;
+    if (!live) return;
     B.dt[slot] = __longlong_as_double((long long)B.dtmin[slot]);
     B.dtmin[slot] = 0x7FF0000000000000ull;
 }
@@ -428,7 +441,7 @@ __global__ void k_batch_commit(BatchDev B) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= B.nactive) return;
     const int slot = B.active[k];
-    if (!item_live(B, slot)) return;
+    if (!step_live(B, slot) || B.flags[slot] != 0) return;
     B.time[slot] += B.dt[slot];
     B.steps[slot] += 1;
 }
@@ -442,6 +455,7 @@ __global__ void k_batch_reset(BatchDev B, const int* __restrict__ list, int n) {
     B.steps[slot] = 0;
     B.dt[slot] = 0.0;
     B.flags[slot] = 0;
+    B.live[slot] = 1;   // for the fill that prepares the problem
 }
 
 // unique-node SoA of one problem <-> its padded block in the arena
@@ -507,6 +521,7 @@ struct sf_batch {
     long long* d_steps = nullptr;
     long long* d_target = nullptr;
     unsigned* d_flags = nullptr;
+    int* d_live = nullptr;
     int4* d_xt = nullptr;
     int4* d_yt = nullptr;
     int* d_active = nullptr;
@@ -551,6 +566,7 @@ BatchDev view(const sf_batch* b) {
     B.steps = b->d_steps;
     B.target = b->d_target;
     B.flags = b->d_flags;
+    B.live = b->d_live;
     return B;
 }
 
@@ -864,12 +880,14 @@ int sf_batch_create(int device, double gamma, size_t node_capacity, int item_cap
     ok = ok && hipMalloc(&b->d_steps, sizeof(long long) * item_capacity) == hipSuccess;
     ok = ok && hipMalloc(&b->d_target, sizeof(long long) * item_capacity) == hipSuccess;
     ok = ok && hipMalloc(&b->d_flags, sizeof(unsigned) * item_capacity) == hipSuccess;
+    ok = ok && hipMalloc(&b->d_live, sizeof(int) * item_capacity) == hipSuccess;
     ok = ok && hipMalloc(&b->d_active, sizeof(int) * item_capacity) == hipSuccess;
     ok = ok && hipMalloc(&b->d_scan, sizeof(int) * item_capacity) == hipSuccess;
     ok = ok && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipStreamCreateWithFlags(&b->copy_stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&b->copy_after, hipEventDisableTiming) == hipSuccess;
     ok = ok && hipMemset(b->d_flags, 0, sizeof(unsigned) * item_capacity) == hipSuccess;
+    ok = ok && hipMemset(b->d_live, 0, sizeof(int) * item_capacity) == hipSuccess;
     ok = ok && hipMemset(b->d_steps, 0, sizeof(long long) * item_capacity) == hipSuccess;
     ok = ok && hipMemset(b->d_target, 0, sizeof(long long) * item_capacity) == hipSuccess;
     // the arena starts zeroed, so padding that is never written stays finite
@@ -898,8 +916,8 @@ int sf_batch_destroy(sf_batch* b) {
         if (p) (void)hipFree(p);
     if (b->S) (void)hipFree(b->S);
     if (b->staging) (void)hipFree(b->staging);
-    void* ptrs[] = {b->d_items, b->d_dt, b->d_dtmin, b->d_time,   b->d_steps, b->d_target,
-                    b->d_flags, b->d_xt, b->d_yt,    b->d_active, b->d_scan};
+    void* ptrs[] = {b->d_items, b->d_dt,   b->d_dtmin, b->d_time,   b->d_steps, b->d_target,
+                    b->d_flags, b->d_live, b->d_xt,    b->d_yt,     b->d_active, b->d_scan};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);

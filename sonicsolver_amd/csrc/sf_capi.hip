@@ -34,8 +34,6 @@ int fail(int code, const std::string& msg) {
             return fail(SF_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kMarchSeg = 32;
-
 }  // namespace
 
 namespace sfamd {
@@ -58,7 +56,13 @@ struct sf_grid {
     double* S = nullptr;
     double* k[3] = {nullptr, nullptr, nullptr};
     Clock* clock = nullptr;
-    unsigned* flags = nullptr;
+    Fault* fault = nullptr;      // device fault record
+    unsigned* flags = nullptr;   // &fault->flags
+    long long seq = 0;           // sequence number of the next step to begin
+    // (step sequence number, buffer holding the state at its start) of the
+    // steps begun since the fault record was last checked
+    std::vector<std::pair<long long, int>> starts;
+    int march_seg = 0;           // nodes per march segment (0: automatic)
     double* dt_hist = nullptr;
     int dt_hist_cap = 0;
     double* stage_io = nullptr;  // unique-node staging for transfers
@@ -147,7 +151,7 @@ void prof_stop(sf_grid* g, int idx) {
 // axis), then the periodic axes.  `all` = 0 skips the periodic axes whose
 // ghosts the final sweep keeps as copies of the interior (G.images): a
 // buffer written by a stage needs no fill for them.
-int fill_ghosts(sf_grid* g, double* q, int all) {
+int fill_ghosts(sf_grid* g, double* q, int all, long long tag) {
     const GridDev& G = g->G;
     const int ev = prof_start(g, SF_KERNEL_GHOST);
     auto plane = [&](int a) {
@@ -160,24 +164,24 @@ int fill_ghosts(sf_grid* g, double* q, int all) {
     for (int a = 0; a < 3; ++a) {
         if (!zero_gradient(a)) continue;
         const unsigned nb = blocks_for(plane(a), 256);
-        if (a == 0) k_bc_copy4<0><<<nb, 256, 0, g->stream>>>(G, q);
-        if (a == 1) k_bc_copy4<1><<<nb, 256, 0, g->stream>>>(G, q);
-        if (a == 2) k_bc_copy4<2><<<nb, 256, 0, g->stream>>>(G, q);
+        if (a == 0) k_bc_copy4<0><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
+        if (a == 1) k_bc_copy4<1><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
+        if (a == 2) k_bc_copy4<2><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
     }
     for (int a = 0; a < 3; ++a) {
         if (!zero_gradient(a)) continue;
         const unsigned nb = blocks_for(plane(a), 256);
-        if (a == 0) k_bc_energy<0><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
-        if (a == 1) k_bc_energy<1><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
-        if (a == 2) k_bc_energy<2><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
+        if (a == 0) k_bc_energy<0><<<nb, 256, 0, g->stream>>>(G, q, g->fault, tag);
+        if (a == 1) k_bc_energy<1><<<nb, 256, 0, g->stream>>>(G, q, g->fault, tag);
+        if (a == 2) k_bc_energy<2><<<nb, 256, 0, g->stream>>>(G, q, g->fault, tag);
     }
     for (int a = 0; a < 3; ++a) {
         if (!G.active[a] || G.bc[a] != SF_BC_PERIODIC) continue;
         if (!all && (G.images & (1 << a))) continue;
         const unsigned nb = blocks_for(plane(a), 256);
-        if (a == 0) k_bc_periodic<0><<<nb, 256, 0, g->stream>>>(G, q);
-        if (a == 1) k_bc_periodic<1><<<nb, 256, 0, g->stream>>>(G, q);
-        if (a == 2) k_bc_periodic<2><<<nb, 256, 0, g->stream>>>(G, q);
+        if (a == 0) k_bc_periodic<0><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
+        if (a == 1) k_bc_periodic<1><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
+        if (a == 2) k_bc_periodic<2><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
     }
     prof_stop(g, ev);
     SF_HIP(hipGetLastError());
@@ -190,7 +194,7 @@ int fill_ghosts(sf_grid* g, double* q, int all) {
 // (dz + dy) + dx rather than the reference's (dx + dy) + dz, which changes
 // the last bit at most.
 int run_sweeps(sf_grid* g, const double* q, const StageArgs& A) {
-    SweepCall c{g->G, q, g->S, A, g->stream, kMarchSeg};
+    SweepCall c{g->G, q, g->S, A, g->stream, g->march_seg};
     int axes[3], na = 0;
     for (int a = 2; a >= 0; --a)
         if (g->G.active[a]) axes[na++] = a;
@@ -225,19 +229,50 @@ int ensure_hist(sf_grid* g, int n) {
     return 0;
 }
 
-int check_flags(sf_grid* g, const char* where) {
-    unsigned h = 0;
-    SF_HIP(hipMemcpyAsync(&h, g->flags, sizeof h, hipMemcpyDeviceToHost, g->stream));
+// Tags of the launches of a step (Fault::tag): the open step's stages and
+// fills, the fill that prepares or commits a step, the CFL scan before it.
+long long stage_tag(const sf_grid* g, int s) { return (g->seq - 1) * 8 + s; }
+long long next_tag(const sf_grid* g, int phase) { return g->seq * 8 + phase; }
+long long open_tag(const sf_grid* g, int phase) { return (g->seq - 1) * 8 + phase; }
+
+// Wait for the grid's work and read its fault record.  With flags raised,
+// every launch after the faulting one returned at entry, so the buffer that
+// held the state at the start of the faulting step still holds it: it
+// becomes the current state again, and the grid's clock (which no step
+// committed after the fault) counts the steps before it.  Reports the
+// 1-based number of the faulting step on the clock and its phase.
+int check_flags(sf_grid* g, const char* where, uint32_t* flags_out = nullptr,
+                long long* step_out = nullptr, int* phase_out = nullptr) {
+    Fault h{};
+    Clock c{};
+    SF_HIP(hipMemcpyAsync(&h, g->fault, sizeof h, hipMemcpyDeviceToHost, g->stream));
+    SF_HIP(hipMemcpyAsync(&c, g->clock, sizeof c, hipMemcpyDeviceToHost, g->stream));
     SF_HIP(hipStreamSynchronize(g->stream));
-    if (h) {
-        char msg[256];
-        std::snprintf(msg, sizeof msg,
-                      "%s: device raised numerical fault flags 0x%x "
-                      "(1 state, 2 face average, 4 TENO cutoff, 8 flux, 16 dt)",
-                      where, h);
-        return fail(SF_E_NUMERIC, msg);
+    if (flags_out) *flags_out = h.flags;
+    if (step_out) *step_out = h.flags ? c.steps + 1 : 0;
+    if (phase_out) *phase_out = h.flags ? (int)(h.tag & 7) : -1;
+    if (!h.flags) {
+        g->starts.clear();
+        return 0;
     }
-    return 0;
+    const long long seq = h.tag >> 3;
+    const int phase = (int)(h.tag & 7);
+    for (const auto& e : g->starts)
+        if (e.first == seq) g->cur = e.second;
+    g->starts.clear();
+    g->step_open = 0;
+    g->ghosts_valid = 0;
+    g->dtmin_valid = 0;
+    char what[32];
+    if (phase < 4) std::snprintf(what, sizeof what, "stage %d", phase + 1);
+    else std::snprintf(what, sizeof what, "%s", phase == kPhaseScan ? "CFL scan" : "ghost fill");
+    char msg[320];
+    std::snprintf(msg, sizeof msg,
+                  "%s: device raised numerical fault flags 0x%x "
+                  "(1 state, 2 face average, 4 TENO cutoff, 8 flux, 16 dt) in step %lld, %s; "
+                  "the grid holds the state at the start of that step",
+                  where, h.flags, c.steps + 1, what);
+    return fail(SF_E_NUMERIC, msg);
 }
 
 // The next step's CFL scan: folded into the last stage's update when
@@ -248,7 +283,8 @@ int scan_dt(sf_grid* g, double cfl) {
         const long long N = (long long)g->G.n[0] * g->G.n[1] * g->G.n[2];
         const unsigned nb = std::min<unsigned>(blocks_for(N, 256), 8192u);
         const int ev = prof_start(g, SF_KERNEL_CFL);
-        k_cfl<<<nb, 256, 0, g->stream>>>(g->G, g->buf[g->cur], cfl, &g->clock->dtmin, g->flags);
+        k_cfl<<<nb, 256, 0, g->stream>>>(g->G, g->buf[g->cur], cfl, &g->clock->dtmin, g->fault,
+                                         next_tag(g, kPhaseScan));
         prof_stop(g, ev);
         SF_HIP(hipGetLastError());
     }
@@ -267,12 +303,14 @@ int set_limits(sf_grid* g, double max_dt, double limit) {
 int do_stage(sf_grid* g, int s, int fold_dt) {
     const int in = g->stage_in[s], out = g->stage_out[s];
     if (s == 0 && !g->ghosts_valid) {
-        if (int rc = fill_ghosts(g, g->buf[in], 1)) return rc;
+        if (int rc = fill_ghosts(g, g->buf[in], 1, stage_tag(g, s))) return rc;
     }
     if (s > 0) {
-        if (int rc = fill_ghosts(g, g->buf[in], 0)) return rc;
+        if (int rc = fill_ghosts(g, g->buf[in], 0, stage_tag(g, s))) return rc;
     }
     StageArgs A{};
+    A.fault_tag = &g->fault->tag;
+    A.tag = stage_tag(g, s);
     A.q0 = g->buf[g->cur];
     A.qout = g->buf[out];
     for (int i = 0; i < 3; ++i) A.k[i] = g->k[i];
@@ -298,6 +336,9 @@ int do_stage(sf_grid* g, int s, int fold_dt) {
     return run_sweeps(g, g->buf[in], A);
 }
 
+// Stage buffers of a step.  No stage writes the buffer that holds the
+// step-start state (a), so that state survives a fault in any stage; the
+// step's result lands in another buffer, which becomes the current one.
 void plan_buffers(sf_grid* g) {
     const int a = g->cur, b = (a + 1) % 3, c = (a + 2) % 3;
     switch (g->desc.time) {
@@ -307,13 +348,13 @@ void plan_buffers(sf_grid* g) {
     case SF_TIME_SSPRK3:
         g->stage_in[0] = a; g->stage_out[0] = b;
         g->stage_in[1] = b; g->stage_out[1] = c;
-        g->stage_in[2] = c; g->stage_out[2] = a;
+        g->stage_in[2] = c; g->stage_out[2] = b;
         break;
     default:
         g->stage_in[0] = a; g->stage_out[0] = b;
         g->stage_in[1] = b; g->stage_out[1] = c;
         g->stage_in[2] = c; g->stage_out[2] = b;
-        g->stage_in[3] = b; g->stage_out[3] = a;
+        g->stage_in[3] = b; g->stage_out[3] = c;
         break;
     }
 }
@@ -324,7 +365,7 @@ int grid_free(sf_grid* g) {
     for (double* p : g->k) if (p) (void)hipFree(p);
     if (g->S) (void)hipFree(g->S);
     if (g->clock) (void)hipFree(g->clock);
-    if (g->flags) (void)hipFree(g->flags);
+    if (g->fault) (void)hipFree(g->fault);
     if (g->dt_hist) (void)hipFree(g->dt_hist);
     if (g->stage_io) (void)hipFree(g->stage_io);
     for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
@@ -372,8 +413,14 @@ int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
         G.active[a] = desc->active[a];
         G.bc[a] = desc->bc[a];
         G.d[a] = desc->d[a];
-        G.images |= (desc->active[a] && desc->bc[a] == SF_BC_PERIODIC) ? (1 << a) : 0;
+        // A node has at most one periodic copy in the ghost layers only
+        // where n >= 2g; thinner periodic axes are filled between stages
+        // (k_bc_periodic takes the node index modulo n).
+        G.images |= (desc->active[a] && desc->bc[a] == SF_BC_PERIODIC && G.n[a] >= 2 * G.g[a])
+                        ? (1 << a) : 0;
     }
+    G.div_line = make_fastdiv((unsigned)G.n[0] + 1u);
+    G.div_n1 = make_fastdiv((unsigned)G.n[1]);
     // A zeroGradient end rewrites the boundary nodes of its axis in the
     // ghost fill after the stage; copies written by the stage would miss
     // that, so the copies are used only where no axis has such an end.
@@ -429,8 +476,9 @@ int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
     if (!rc) rc = alloc(&g->stage_io, sizeof(double) * 5 * N);
     if (!rc && hipMalloc(&g->clock, sizeof(Clock)) != hipSuccess)
         rc = fail(SF_E_MEMORY, "sf_grid_create: clock allocation failed");
-    if (!rc && hipMalloc(&g->flags, sizeof(unsigned)) != hipSuccess)
-        rc = fail(SF_E_MEMORY, "sf_grid_create: flag allocation failed");
+    if (!rc && hipMalloc(&g->fault, sizeof(Fault)) != hipSuccess)
+        rc = fail(SF_E_MEMORY, "sf_grid_create: fault record allocation failed");
+    if (!rc) g->flags = &g->fault->flags;
     if (!rc && hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(SF_E_HIP, "sf_grid_create: stream creation failed");
     if (rc) {
@@ -444,9 +492,9 @@ int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
     c.dtmin = kInfBits;
     c.max_dt = std::numeric_limits<double>::max();
     c.limit = std::numeric_limits<double>::max();
-    unsigned zero = 0;
+    Fault f{0u, 0u, -1};
     if (hipMemcpy(g->clock, &c, sizeof c, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(g->flags, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(g->fault, &f, sizeof f, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(g->buf[0], 0, state) != hipSuccess ||
         hipMemset(g->buf[1], 0, state) != hipSuccess ||
         hipMemset(g->buf[2], 0, state) != hipSuccess ||
@@ -476,10 +524,13 @@ int sf_grid_stream(sf_grid* g, void** stream) {
     return 0;
 }
 
+// A new state starts with a clear fault record.
 static int upload_from(sf_grid* g, const double* dsrc) {
     const GridDev& G = g->G;
     const long long N = (long long)G.n[0] * G.n[1] * G.n[2];
     const unsigned nb = std::min<unsigned>(blocks_for(N, 256), 65535u);
+    k_clear_fault<<<1, 64, 0, g->stream>>>(g->fault);
+    g->starts.clear();
     k_pack<<<nb, 256, 0, g->stream>>>(G, dsrc, g->buf[g->cur], 1);
     // ends without a condition: every state buffer's ghost layers there hold
     // the uploaded end nodes from now on
@@ -546,7 +597,7 @@ int sf_step_prepare(sf_grid* g) {
     if (int rc = set_device(g)) return rc;
     if (g->step_open) return fail(SF_E_STATE, "sf_step_prepare: a step is open");
     if (!g->ghosts_valid) {
-        if (int rc = fill_ghosts(g, g->buf[g->cur], 1)) return rc;
+        if (int rc = fill_ghosts(g, g->buf[g->cur], 1, next_tag(g, kPhaseFill))) return rc;
         g->ghosts_valid = 1;
     }
     return 0;
@@ -561,7 +612,7 @@ int sf_compute_dt(sf_grid* g, double cfl, double max_dt, double limit, double* d
     if (g->step_open) return fail(SF_E_STATE, "sf_compute_dt: a step is open");
     if (int rc = set_limits(g, max_dt, limit)) return rc;
     if (int rc = scan_dt(g, cfl)) return rc;
-    k_finalize_dt<<<1, 64, 0, g->stream>>>(g->clock, nullptr, 0);
+    k_finalize_dt<<<1, 64, 0, g->stream>>>(g->clock, nullptr, 0, g->flags);
     SF_HIP(hipGetLastError());
     g->step_cfl = cfl;
     if (dt_out) {
@@ -587,6 +638,8 @@ int sf_step_begin(sf_grid* g) {
     if (!g) return fail(SF_E_ARG, "sf_step_begin: null grid");
     if (g->step_open) return fail(SF_E_STATE, "sf_step_begin: a step is already open");
     plan_buffers(g);
+    g->starts.push_back({g->seq, g->cur});
+    ++g->seq;
     g->step_open = 1;
     g->next_stage = 0;
     return 0;
@@ -614,8 +667,8 @@ int sf_step_commit(sf_grid* g) {
         return fail(SF_E_STATE, "sf_step_commit: stages incomplete");
     if (int rc = set_device(g)) return rc;
     g->cur = g->stage_out[nstages(g->desc.time) - 1];
-    if (int rc = fill_ghosts(g, g->buf[g->cur], 0)) return rc;
-    k_commit_clock<<<1, 64, 0, g->stream>>>(g->clock);
+    if (int rc = fill_ghosts(g, g->buf[g->cur], 0, open_tag(g, kPhaseFill))) return rc;
+    k_commit_clock<<<1, 64, 0, g->stream>>>(g->clock, g->flags);
     SF_HIP(hipGetLastError());
     g->ghosts_valid = 1;
     g->dtmin_valid = g->fold_cfl;
@@ -640,7 +693,8 @@ int sf_run(sf_grid* g, int nsteps, double cfl, double max_dt, double* dt_seq) {
     for (int s = 0; s < nsteps; ++s) {
         if (int rc = sf_step_prepare(g)) return rc;
         if (int rc = scan_dt(g, cfl)) return rc;
-        k_finalize_dt<<<1, 64, 0, g->stream>>>(g->clock, dt_seq ? g->dt_hist : nullptr, s);
+        k_finalize_dt<<<1, 64, 0, g->stream>>>(g->clock, dt_seq ? g->dt_hist : nullptr, s,
+                                               g->flags);
         SF_HIP(hipGetLastError());
         if (int rc = sf_step_begin(g)) return rc;
         for (int st = 0; st < nstages(g->desc.time); ++st)
@@ -679,6 +733,8 @@ int sf_residual(sf_grid* g, double* soa_out) {
     A.time = 3;  // residual only
     A.dt = &g->clock->dt;
     A.dtmin = &g->clock->dtmin;
+    A.fault_tag = &g->fault->tag;
+    A.tag = next_tag(g, 0);
     if (int rc = run_sweeps(g, g->buf[g->cur], A)) return rc;
     const long long N = (long long)g->G.n[0] * g->G.n[1] * g->G.n[2];
     const unsigned nb = std::min<unsigned>(blocks_for(N, 256), 65535u);
@@ -698,9 +754,24 @@ int sf_error_flags(sf_grid* g, uint32_t* flags, int clear) {
     SF_HIP(hipStreamSynchronize(g->stream));
     *flags = h;
     if (clear) {
-        const unsigned zero = 0;
-        SF_HIP(hipMemcpy(g->flags, &zero, sizeof zero, hipMemcpyHostToDevice));
+        k_clear_fault<<<1, 64, 0, g->stream>>>(g->fault);
+        SF_HIP(hipGetLastError());
+        SF_HIP(hipStreamSynchronize(g->stream));
     }
+    return 0;
+}
+
+int sf_fault_check(sf_grid* g, uint32_t* flags, long long* step, int* phase) {
+    if (!g) return fail(SF_E_ARG, "sf_fault_check: null grid");
+    if (int rc = set_device(g)) return rc;
+    return check_flags(g, "sf_fault_check", flags, step, phase);
+}
+
+int sf_grid_set_march_segment(sf_grid* g, int nodes) {
+    if (!g) return fail(SF_E_ARG, "sf_grid_set_march_segment: null grid");
+    if (nodes != 0 && (nodes < 1 || nodes > 4096))
+        return fail(SF_E_ARG, "sf_grid_set_march_segment: 0 (automatic) or 1..4096 nodes");
+    g->march_seg = nodes;
     return 0;
 }
 

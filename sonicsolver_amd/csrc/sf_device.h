@@ -8,6 +8,31 @@
 
 namespace sfamd {
 
+// Unsigned 32-bit division by a divisor fixed for a whole launch, as a
+// multiply-high, an add and a shift (Granlund and Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, section 4): with
+// l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
+// floor(x / d) = (umulhi(x, m) + x) >> l for every x < 2^32 (the sum is
+// formed in 64 bits).  The host forms (m, l) with make_fastdiv.
+struct FastDiv {
+    unsigned m;
+    unsigned l;
+};
+
+inline FastDiv make_fastdiv(unsigned d) {
+    unsigned l = 0;
+    while (l < 32 && (1ull << l) < d) ++l;
+    FastDiv f;
+    f.m = (unsigned)((((1ull << l) - d) << 32) / d + 1);
+    f.l = l;
+    return f;
+}
+
+__device__ __forceinline__ unsigned fast_div(unsigned x, FastDiv f) {
+    const unsigned t = __umulhi(x, f.m);
+    return (unsigned)(((unsigned long long)t + x) >> f.l);
+}
+
 struct GridDev {
     int n[3];
     int g[3];
@@ -31,6 +56,9 @@ struct GridDev {
     // Nodes the stage advances along each axis: solved_lo[a] .. solved_hi[a]
     // (a zeroGradient end's node is excluded; see solved_node).
     int solved_lo[3], solved_hi[3];
+    // The x sweep's face -> (line, node) and line -> (j, k) divisions:
+    // by n0 + 1 faces per line and by n1 lines per z plane.
+    FastDiv div_line, div_n1;
 };
 
 enum { kCopyS = 0, kAccumS = 1, kFinal = 2 };
@@ -48,7 +76,23 @@ struct StageArgs {
     int time;                // 0 Euler, 1 SSPRK3, 2 RK4
     int stage;
     int reduce_dt;           // fold the next step's CFL scan into this update
+    long long* fault_tag;    // where the first fault is recorded (may be null)
+    long long tag;           // this launch's tag (step sequence * 8 + phase)
 };
+
+// Fault record of a grid: the flags raised since they were last cleared,
+// and the tag of the launch that raised the first of them: step sequence
+// number * 8 + phase (0..3 the stage, its ghost fill included; 6 the ghost
+// fill at the start or commit of the step; 7 the CFL scan before it).
+// Once flags are raised, every later launch of a stage, fill or clock
+// kernel returns at entry, so the state at the start of the faulting step
+// stays as it was.
+struct Fault {
+    unsigned flags;
+    unsigned pad;
+    long long tag;
+};
+enum { kPhaseFill = 6, kPhaseScan = 7 };
 
 __device__ __forceinline__ long long node_index(const GridDev& G, int i, int j, int k) {
     return (long long)(k + G.g[2]) * G.stride[2] + (long long)(j + G.g[1]) * G.stride[1]
@@ -109,12 +153,18 @@ __device__ __forceinline__ void raise_flags(unsigned f, unsigned* flags) {
     if (f) atomicOr(flags, f);
 }
 
+// raise_flags, recording `tag` if these are the first flags raised.
+__device__ __forceinline__ void raise_fault(unsigned f, unsigned* flags, long long* tag_out,
+                                            long long tag) {
+    if (f && atomicOr(flags, f) == 0u && tag_out) *tag_out = tag;
+}
+
 // Node (i, j, k) at `id` was just written with qn: copy it into the ghost
 // positions of the axes in G.images that hold its periodic copies (node c
 // of an axis with n nodes and g ghost layers is ghost c + n if c < g and
-// ghost c - n if c >= n - g; n >= 4 >= g, so there is at most one copy per
-// axis).  The sweeps read ghosts only along their own axis, so no corner
-// copies are needed.
+// ghost c - n if c >= n - g; the host sets an axis's bit only where
+// n >= 2g, so there is at most one copy per axis).  The sweeps read ghosts
+// only along their own axis, so no corner copies are needed.
 __device__ __forceinline__ void write_images(const GridDev& G, double* __restrict__ out, int i,
                                              int j, int k, long long id, const double qn[5]) {
     const int c[3] = {i, j, k};
@@ -181,7 +231,10 @@ __device__ __forceinline__ void stage_update_values(const GT& G, const StageArgs
         A.qout[v * G.P + id] = qn[v];
     }
     // validateEquationState (SF_equationState.h:25-57): rho, p > 0, finite
-    const double ke = 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) / qn[0];
+    // (p has the sign of the internal energy; with the refined reciprocal
+    // of the density instead of a division, that sign can differ only where
+    // the internal energy is within an ulp of the kinetic energy)
+    const double ke = 0.5 * (qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]) * frcp(qn[0]);
     const double eint = qn[4] - ke;
     if (!(qn[0] > 0.0) || !(eint > 0.0) || !isfinite(eint)) flags |= kFlagState;
     if (A.reduce_dt) dt_local = local_dt(G, qn, A.cfl, flags);

@@ -8,7 +8,8 @@
 //
 //   ghost fill (O(surface))      k_bc_copy4 / k_bc_energy / k_bc_periodic
 //   x sweep                      k_sweep_x:     one lane per x face, the
-//                                               flux difference via LDS
+//                                               flux difference by a
+//                                               cross-lane shift
 //   y / z sweeps                 k_sweep_march: one lane per (x, column
 //                                               segment), marching along the
 //                                               axis with the stencil window
@@ -32,7 +33,9 @@ namespace sfamd {
 // Standalone CFL scan (first step, or sf_compute_dt).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_cfl(GridDev G, const double* __restrict__ q, double cfl,
-                                             unsigned long long* dtmin, unsigned* flags_out) {
+                                             unsigned long long* dtmin, Fault* fault,
+                                             long long tag) {
+    if (fault->flags) return;
     const long long N = (long long)G.n[0] * G.n[1] * G.n[2];
     unsigned flags = 0;
     double best = INFINITY;
@@ -48,12 +51,12 @@ __global__ __launch_bounds__(256) void k_cfl(GridDev G, const double* __restrict
         for (int v = 0; v < 5; ++v) qq[v] = q[v * G.P + id];
         best = fmin(best, local_dt(G, qq, cfl, flags));
     }
-    raise_flags(flags, flags_out);
+    raise_fault(flags, &fault->flags, &fault->tag, tag);
     wave_min_atomic(best, dtmin);
 }
 
-__global__ void k_finalize_dt(Clock* c, double* dt_hist, int slot) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void k_finalize_dt(Clock* c, double* dt_hist, int slot, const unsigned* flags) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || *flags) return;
     double dt = __longlong_as_double((long long)c->dtmin);
     dt = fmin(dt, c->max_dt);
     dt = fmin(dt, c->limit);
@@ -74,8 +77,8 @@ __global__ void k_set_limits(Clock* c, double max_dt, double limit, int reset_sc
     }
 }
 
-__global__ void k_commit_clock(Clock* c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+__global__ void k_commit_clock(Clock* c, const unsigned* flags) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !*flags) {
         c->time += c->dt;
         c->steps += 1;
     }
@@ -100,7 +103,9 @@ __device__ __forceinline__ bool plane_point(const GridDev& G, long long t, int p
 // zeroGradient density + momentum (SF_zeroGradient.cpp:88-130) on the
 // zeroGradient ends of the axis; the ghost depth is G.g[AXIS].
 template <int AXIS>
-__global__ __launch_bounds__(256) void k_bc_copy4(GridDev G, double* __restrict__ q) {
+__global__ __launch_bounds__(256) void k_bc_copy4(GridDev G, double* __restrict__ q,
+                                                  const unsigned* flags) {
+    if (*flags) return;
     int p[3], side;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (!plane_point<AXIS>(G, t, p, side)) return;
@@ -120,7 +125,8 @@ __global__ __launch_bounds__(256) void k_bc_copy4(GridDev G, double* __restrict_
 // zeroGradient pressure -> energy (SF_thermodynamicClosure.cpp:219-281).
 template <int AXIS>
 __global__ __launch_bounds__(256) void k_bc_energy(GridDev G, double* __restrict__ q,
-                                                   unsigned* flags_out) {
+                                                   Fault* fault, long long tag) {
+    if (fault->flags) return;
     int p[3], side;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (!plane_point<AXIS>(G, t, p, side)) return;
@@ -132,7 +138,7 @@ __global__ __launch_bounds__(256) void k_bc_energy(GridDev G, double* __restrict
     const double ps = pressure_of(q, G.P, s, G.gamma);
     q[4 * G.P + b] = energy_from_pressure(q, G.P, b, G.gamma, ps);
     const double pb = pressure_of(q, G.P, b, G.gamma);
-    if (!(ps > 0.0) || !(pb > 0.0)) atomicOr(flags_out, (unsigned)kFlagState);
+    if (!(ps > 0.0) || !(pb > 0.0)) raise_fault(kFlagState, &fault->flags, &fault->tag, tag);
     for (int l = 1; l <= G.g[AXIS]; ++l)
         q[4 * G.P + b + l * dir] = energy_from_pressure(q, G.P, b + l * dir, G.gamma, pb);
 }
@@ -158,7 +164,9 @@ __global__ __launch_bounds__(256) void k_bc_hold(GridDev G, const double* __rest
 
 // Periodic: ghost layer l on either side takes the node (index mod n).
 template <int AXIS>
-__global__ __launch_bounds__(256) void k_bc_periodic(GridDev G, double* __restrict__ q) {
+__global__ __launch_bounds__(256) void k_bc_periodic(GridDev G, double* __restrict__ q,
+                                                     const unsigned* flags) {
+    if (*flags) return;
     int p[3], side;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (!plane_point<AXIS>(G, t, p, side)) return;
@@ -173,6 +181,16 @@ __global__ __launch_bounds__(256) void k_bc_periodic(GridDev G, double* __restri
                                    : base + (long long)(((n - l) % n + n) % n) * step;
 #pragma unroll
         for (int v = 0; v < 5; ++v) q[v * G.P + dst] = q[v * G.P + src];
+    }
+}
+
+// Clear a fault record (a kernel rather than a memset, so that the clear
+// can also be captured in a graph).
+__global__ void k_clear_fault(Fault* fault) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        fault->flags = 0u;
+        fault->pad = 0u;
+        fault->tag = -1;
     }
 }
 

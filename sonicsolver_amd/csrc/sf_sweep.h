@@ -51,11 +51,23 @@ int launch_sweep_fm(const SweepCall& c, int axis) {
         const unsigned other = (unsigned)(axis == 1 ? G.n[2] : G.n[1]);
         const unsigned cols = (unsigned)((G.n[0] + kMarchBlock - 1) / kMarchBlock);
         // Each segment re-reads the stencil window and evaluates one extra
-        // face at its start, so segments twice the base length are used
-        // wherever that still leaves 8192 blocks or more to fill the chip
-        // (512^3: 64 nodes).
+        // face at its start, so segments of 64 nodes are used wherever that
+        // still leaves 8192 one-wave blocks or more to fill the chip
+        // (512^3); where 32-node segments leave fewer than 4096 blocks
+        // (2048^2 in 2D: 2048), the segment is halved down to 8 nodes until
+        // there are 4096, four waves per SIMD.  A face's arithmetic does not
+        // depend on the segment length, so neither does the result.
+        // c.seg > 0 fixes the length instead.
+        auto blocks = [&](int s) {
+            return (long long)cols * other * ((G.n[axis] + s - 1) / s);
+        };
         int seg = c.seg;
-        if ((long long)cols * other * ((G.n[axis] + 2 * seg - 1) / (2 * seg)) >= 8192) seg *= 2;
+        if (seg <= 0) {
+            seg = 32;
+            if (blocks(2 * seg) >= 8192) seg *= 2;
+            else
+                while (seg > 8 && blocks(seg) < 4096) seg /= 2;
+        }
         const dim3 grid(cols, other, (unsigned)((G.n[axis] + seg - 1) / seg));
         if (axis == 1)
             k_sweep_march<1, SCHEME, FLUX, MODE><<<grid, kMarchBlock, 0, c.stream>>>(

@@ -50,6 +50,7 @@ template <int SCHEME, int FLUX, int TIME>
 __global__ __launch_bounds__(kXBlock)
 __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const double* __restrict__ q,
                                                      const double* __restrict__ S, StageArgs A) {
+    if (*A.flags) return;   // a fault was raised: keep the step-start state
     // face numbers fit 32 bits: the host rejects grids with 2^32 - 2^8
     // faces or more along x
     const int t = threadIdx.x;
@@ -68,9 +69,10 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
     for (unsigned T = first; T < tiles; T += gridDim.x) {
         const unsigned f = T * (kXBlock - 1) + t;
         const bool face = f < faces;
-        const unsigned line = face ? f / per_line : 0;
+        const unsigned line = face ? fast_div(f, G.div_line) : 0;
         const int a = face ? (int)(f - line * per_line) - 1 : 0;  // lower node of the face
-        const int j = (int)(line % (unsigned)G.n[1]), k = (int)(line / (unsigned)G.n[1]);
+        const unsigned kz = fast_div(line, G.div_n1);
+        const int j = (int)(line - kz * (unsigned)G.n[1]), k = (int)kz;
         double f5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         // The final sweep's stage operands for the node this lane updates
         // (the lower node of its face): loaded before the face arithmetic,
@@ -122,7 +124,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
             dt_local = fmin(dt_local, dl);
         }
     }
-    raise_flags(flags, A.flags);
+    raise_fault(flags, A.flags, A.fault_tag, A.tag);
     if (A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);
 }
 
@@ -151,6 +153,7 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
     const int a0 = blockIdx.z * seg;
     const int a1 = min(a0 + seg, G.n[AXIS]);
     static_assert(MODE == kCopyS || MODE == kAccumS, "a march writes or accumulates S; the x sweep applies the update");
+    if (*A.flags) return;   // a fault was raised: keep the step-start state
     unsigned flags = 0;
     const double lw = (FLUX == 5) ? *A.dt / G.d[AXIS] : 0.0;   // Lax-Wendroff dt / spacing
     if (i < G.n[0]) {
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
             for (int v = 0; v < 5; ++v) S[v * G.P + id] = s5[v];
         }
     }
-    raise_flags(flags, A.flags);
+    raise_fault(flags, A.flags, A.fault_tag, A.tag);
 }
 
 }  // namespace sfamd

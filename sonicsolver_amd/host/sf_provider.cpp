@@ -326,8 +326,13 @@ void FlowConservativeProvider::bindExplicitOps(OpRegistry& ops) {
             throw std::runtime_error("explicit.stage.execute outside its stage loop.");
         check(sf_stage(grid_, c.stageIndex));
     });
+    // The reference throws at the first non-physical stage
+    // (SF_equationState.h:25-57); here the step's fault record is checked
+    // once the step is committed, before time.commit, so a faulting step is
+    // never counted and the grid keeps the state at its start.
     ops.bind(OpIds::FlowStepCommit, p, [this](const ExecutionContext&) {
         check(sf_step_commit(grid_));
+        check(sf_fault_check(grid_, nullptr, nullptr, nullptr));
     });
     ops.bind(OpIds::TimeCommit, p, [this](const ExecutionContext&) {
         state_.time += state_.dt;

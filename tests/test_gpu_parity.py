@@ -116,3 +116,44 @@ def test_step_by_step_ops_match_run():
             g.commit()
         b = g.download()
     np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,bc,scheme,flux,time", [
+    ((40, 4, 1), ("transmissive", "periodic", "transmissive"), "weno5", "hllc", "ssprk3"),
+    ((40, 5, 1), ("transmissive", "periodic", "transmissive"), "weno5", "rusanov", "ssprk3"),
+    ((4, 24, 1), ("periodic", "periodic", "transmissive"), "weno5", "hllc", "rk4"),
+    ((5, 24, 1), ("periodic", "periodic", "transmissive"), "teno5", "rusanov", "ssprk3"),
+    ((30, 5, 1), ("periodic", "periodic", "transmissive"), "weno7", "rusanov", "ssprk3"),
+    ((24, 20, 5), ("periodic",) * 3, "weno5", "hllc", "rk4"),
+    ((20, 18, 4), ("periodic",) * 3, "weno7", "roe", "euler"),
+])
+def test_thin_periodic_axes(n, bc, scheme, flux, time):
+    """Periodic axes with fewer than twice the ghost depth of nodes: a node
+    then has copies in both ghost layers, so such an axis is filled between
+    stages instead of by the final sweep's copies."""
+    active = tuple(1 if x > 1 else 0 for x in n)
+    q = wl.smooth_field(n, seed=17)
+    kw = dict(n=n, active=active, bc=bc, d=(0.1, 0.12, 0.09), scheme=scheme, flux=flux,
+              time=time)
+    out, dts, ref, ref_dts = _both(kw, q, 6, 0.4)
+    np.testing.assert_allclose(dts, ref_dts, rtol=1e-12, atol=0)
+    assert_close(out, ref, what=f"thin periodic {n} {scheme}/{flux}/{time}")
+
+
+@pytest.mark.parametrize("case", ["vortex_2d", "tgv_3d"])
+def test_march_segment_length_does_not_change_the_result(case):
+    """The y / z marches evaluate every face by the same instructions
+    wherever their segments start, so the state after a few steps is the
+    same bit for bit for every segment length (the automatic choice of the
+    2048^2 vortex is 16 nodes, of the 512^3 field 64)."""
+    kw, q = wl.vortex_2d(256) if case == "vortex_2d" else wl.tgv_3d(48)
+    outs = []
+    for seg in (0, 8, 16, 32, 64, 5):
+        print(f"case {case} march segment {seg}", flush=True)
+        with Grid(make_desc(**kw)) as g:
+            g.set_march_segment(seg)
+            g.upload(q)
+            g.run(3, 0.5 if case == "vortex_2d" else 0.4)
+            outs.append(g.download())
+    for seg, o in zip((8, 16, 32, 64, 5), outs[1:]):
+        assert np.array_equal(o, outs[0]), seg
