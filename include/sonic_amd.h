@@ -188,6 +188,26 @@ int sf_synchronize(sf_grid *grid);
 /* The HIP stream (hipStream_t) the grid queues its work on. */
 int sf_grid_stream(sf_grid *grid, void **stream);
 
+/* ---- point evaluations of the face arithmetic ------------------------------
+ * The device functions the sweeps use, applied to caller-given data, for
+ * checking them against a CPU restatement.  Arrays are host memory,
+ * array-of-structures: state i at [5 i .. 5 i + 4]; flags (may be NULL)
+ * receives each point's SF_FLAG_* bits.  Blocking. */
+/* The face flux of `flux` (SF_FLUX_*) between qL[i] and qR[i] on a face
+ * whose unit normal is e_axis: the FaceFlux interface of the reference's
+ * recipes (RusanovEOS::faceFlux, SF_rusanovEOS.cpp:47-77; StegerWarming,
+ * LaxFriedrichs, Roe and LaxWendroff fluxes in methods/numerics/Flux;
+ * HLLC has no reference counterpart).  lw = dt / spacing (Lax-Wendroff
+ * only). */
+int sf_riemann_flux(int device, int flux, int axis, double gamma, double lw, int count,
+                    const double *qL, const double *qR, double *out, uint32_t *flags);
+/* Flux::characteristicWENOFaceFlux<6> (SF_WENO.h:185-245) of the six-node
+ * stencils[i] (nodes at offsets -2..3 around the face, [(6 i + s) 5 + v])
+ * on a face of unit normal e_axis and the given area; WENO5 / TENO5
+ * (SF_SCHEME_*) with the Rusanov or HLLC flux. */
+int sf_face_flux(int device, int scheme, int flux, int axis, double gamma, double area,
+                 int count, const double *stencils, double *out, uint32_t *flags);
+
 /* ---- kernel timing -----------------------------------------------------------
  * No reference counterpart (the reference has no timers).  When enabled,
  * every launch of the kernels below is bracketed by HIP events recorded on

@@ -82,6 +82,11 @@ def _declare(lib):
         "sf_fault_check": [_VP, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_longlong),
                            ctypes.POINTER(ctypes.c_int)],
         "sf_grid_set_march_segment": [_VP, ctypes.c_int],
+        "sf_riemann_flux": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                            ctypes.c_double, ctypes.c_int, _P, _P, _P,
+                            ctypes.POINTER(ctypes.c_uint32)],
+        "sf_face_flux": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                         ctypes.c_double, ctypes.c_int, _P, _P, ctypes.POINTER(ctypes.c_uint32)],
         "sf_synchronize": [_VP],
         "sf_profile_enable": [_VP, ctypes.c_int],
         "sf_profile_read": [_VP, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_longlong)],
@@ -388,6 +393,35 @@ class Solver:
         _check(load().sf_solver_dt_history(self._h, out.ctypes.data_as(_P), n.value,
                                            ctypes.byref(n)))
         return out
+
+
+def riemann_flux(flux, axis: int, qL, qR, gamma: float = 1.4, lw: float = 0.0, device: int = 0):
+    """Device face fluxes of state pairs (count x 5 arrays); (flux, flags)."""
+    a = np.ascontiguousarray(qL, dtype=np.float64).reshape(-1, 5)
+    b = np.ascontiguousarray(qR, dtype=np.float64).reshape(-1, 5)
+    out = np.empty_like(a)
+    flags = np.zeros(len(a), dtype=np.uint32)
+    f = FLUXES[flux.lower()] if isinstance(flux, str) else int(flux)
+    _check(load().sf_riemann_flux(device, f, int(axis), float(gamma), float(lw), len(a),
+                                  a.ctypes.data_as(_P), b.ctypes.data_as(_P),
+                                  out.ctypes.data_as(_P),
+                                  flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+    return out, flags
+
+
+def face_flux(scheme, flux, axis: int, stencils, area: float = 1.0, gamma: float = 1.4,
+              device: int = 0):
+    """Device characteristic WENO face fluxes of six-node stencils
+    (count x 6 x 5); (flux, flags)."""
+    st = np.ascontiguousarray(stencils, dtype=np.float64).reshape(-1, 6, 5)
+    out = np.empty((len(st), 5))
+    flags = np.zeros(len(st), dtype=np.uint32)
+    sc = SCHEMES[scheme.lower()] if isinstance(scheme, str) else int(scheme)
+    f = FLUXES[flux.lower()] if isinstance(flux, str) else int(flux)
+    _check(load().sf_face_flux(device, sc, f, int(axis), float(gamma), float(area), len(st),
+                               st.ctypes.data_as(_P), out.ctypes.data_as(_P),
+                               flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+    return out, flags
 
 
 def describe_flags(flags: int) -> str:

@@ -18,7 +18,7 @@ ROOT = PKG.parent
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SONIC_AMD_ARCH", "gfx950")
 
-SOURCES = [PKG / "csrc" / "sf_capi.hip", PKG / "csrc" / "sf_batch.hip",
+SOURCES = [PKG / "csrc" / "sf_capi.hip", PKG / "csrc" / "sf_batch.hip", PKG / "csrc" / "sf_diag.hip",
            PKG / "csrc" / "sf_sweep_weno5.hip", PKG / "csrc" / "sf_sweep_teno5.hip",
            PKG / "csrc" / "sf_sweep_weno3.hip", PKG / "csrc" / "sf_sweep_weno7.hip",
            PKG / "host" / "sf_provider.cpp", PKG / "host" / "sf_server.cpp",

@@ -67,20 +67,24 @@ def test_multid_smooth(n, active, bc, scheme, flux, time):
     assert_close(out, ref, what=f"{n} {bc} {scheme}/{flux}/{time}")
 
 
-def test_vortex_2d_reduced():
-    """2D isentropic vortex (TENO5, HLLC, SSPRK3, CFL 0.5, periodic) on 128^2."""
+def test_vortex_2d_reduced_grid_stated_steps():
+    """The 2D configuration (isentropic vortex with density noise, TENO5,
+    HLLC, SSPRK3, CFL 0.5, periodic) on 128^2 for its full 100 steps.  On
+    this smooth field the reference built with and without FMA contraction
+    agrees to 6e-15 after the 100 steps, so TENO5 is held to 1e-11 here."""
     kw, q = wl.vortex_2d(128)
-    out, dts, ref, ref_dts = _both(kw, q, 20, 0.5)
+    out, dts, ref, ref_dts = _both(kw, q, 100, 0.5)
     np.testing.assert_allclose(dts, ref_dts, rtol=1e-12, atol=0)
-    assert_close(out, ref, what="vortex 128")
+    assert_close(out, ref, what="vortex 128, 100 steps")
 
 
-def test_tgv_3d_reduced():
-    """3D periodic TGV field (WENO5, HLLC, SSPRK3, CFL 0.4) on 32^3."""
+def test_tgv_3d_reduced_grid_stated_steps():
+    """The 3D configuration (periodic Taylor-Green-type field, WENO5, HLLC,
+    SSPRK3, CFL 0.4) on 32^3 for its full 50 steps."""
     kw, q = wl.tgv_3d(32)
-    out, dts, ref, ref_dts = _both(kw, q, 5, 0.4)
+    out, dts, ref, ref_dts = _both(kw, q, 50, 0.4)
     np.testing.assert_allclose(dts, ref_dts, rtol=1e-12, atol=0)
-    assert_close(out, ref, what="tgv 32")
+    assert_close(out, ref, what="tgv 32, 50 steps")
 
 
 def test_residual_matches_oracle():

@@ -1,0 +1,116 @@
+"""The reference-side binding (sonicsolver_amd/integration/) compiles
+against the reference's own headers, and its two patches apply to the
+reference tree and compile: the HLLC recipe registration
+(hllc_recipes.patch: FluxSplitter::HLLC, the weno3/weno5/teno5/weno7 HLLC
+recipe ids and names, parseFluxSplitter, resolveConvectionTermRecipe) and
+the FlowOperations call site (flowOperations.patch).  A probe linked
+against the patched headers resolves the new recipe names as the reference
+resolves its own.  Runs where the reference sources are present (not on
+the GPU box); the patched copies live in a temporary directory."""
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+REF = Path("/root/reference")
+SRC = REF / "src"
+INTEG = ROOT / "sonicsolver_amd" / "integration"
+
+pytestmark = pytest.mark.skipif(not SRC.is_dir(), reason="the reference sources are not here")
+
+
+def ref_includes(first=()):
+    """The include order of oracle/Makefile (core/field and core/state before
+    the rest; infrastructure/io, whose SF_field.h would shadow core's, and
+    the GUI left out), with `first` ahead of it."""
+    dirs = [str(d) for d in first] + [str(SRC), str(SRC / "core" / "field"),
+                                       str(SRC / "core" / "state")]
+    for d in sorted(p for p in SRC.rglob("*") if p.is_dir()):
+        s = str(d)
+        if "/infrastructure/io" in s or "/app/gui" in s:
+            continue
+        dirs.append(s)
+    return [f"-I{d}" for d in dirs]
+
+
+def gxx(args, cwd=None):
+    return subprocess.run(["g++", "-std=c++17", *args], capture_output=True, text=True, cwd=cwd)
+
+
+def test_binding_compiles_against_the_reference():
+    r = gxx(["-fsyntax-only", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", *ref_includes(),
+             str(INTEG / "SF_deviceExplicitOps.cpp")])
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert "SF_deviceExplicitOps" not in r.stderr, r.stderr[-3000:]
+
+
+@pytest.fixture(scope="module")
+def patched(tmp_path_factory):
+    tree = tmp_path_factory.mktemp("patched")
+    files = ["src/core/config/types/SF_termRecipe.h",
+             "src/app/application/model/parsing/SF_numericsParser.h",
+             "src/solver/algorithm/SF_flowOperations.h",
+             "src/solver/algorithm/SF_flowOperations.cpp"]
+    for f in files:
+        (tree / f).parent.mkdir(parents=True, exist_ok=True)
+        shutil.copy(REF / f, tree / f)
+    for p in ("hllc_recipes.patch", "flowOperations.patch"):
+        r = subprocess.run(["patch", "-p1", "--forward", "-i", str(INTEG / p)], cwd=tree,
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout + r.stderr
+    # the binding sits in src/solver/algorithm beside FlowOperations
+    for f in ("SF_deviceExplicitOps.h", "SF_deviceExplicitOps.cpp"):
+        shutil.copy(INTEG / f, tree / "src" / "solver" / "algorithm" / f)
+    return tree
+
+
+def _shadow(tree):
+    return [tree / "src", tree / "src" / "core" / "config" / "types",
+            tree / "src" / "app" / "application" / "model" / "parsing",
+            tree / "src" / "solver" / "algorithm"]
+
+
+def test_patched_tree_compiles(patched):
+    inc = [f"-I{ROOT / 'include'}", *ref_includes(_shadow(patched))]
+    for tu in ("src/solver/algorithm/SF_deviceExplicitOps.cpp",
+               "src/solver/algorithm/SF_flowOperations.cpp"):
+        r = gxx(["-fsyntax-only", "-w", *inc, str(patched / tu)])
+        assert r.returncode == 0, (tu, r.stderr[-3000:])
+
+
+def test_patched_recipes_resolve(patched, tmp_path):
+    probe = tmp_path / "probe.cpp"
+    probe.write_text(r'''
+#include "SF_numericsParser.h"
+#include <cstdio>
+#include <cstring>
+using namespace SF::FDM;
+int main() {
+    int bad = 0;
+    const char* names[] = {"weno3HLLC", "weno5HLLC", "teno5HLLC", "weno7HLLC"};
+    const ConvectionScheme schemes[] = {ConvectionScheme::WENO3, ConvectionScheme::WENO5,
+                                        ConvectionScheme::TENO5, ConvectionScheme::WENO7};
+    const int halo[] = {2, 3, 3, 4};
+    for (int k = 0; k < 4; ++k) {
+        const TermRecipe r = resolveConvectionTermRecipe(names[k]);
+        bad += r.flux() != FluxSplitter::HLLC || r.convection() != schemes[k]
+            || r.haloWidth() != halo[k] || std::strcmp(toString(r.id()), names[k]) != 0;
+    }
+    bad += parseFluxSplitter("HLLC") != FluxSplitter::HLLC;
+    bad += std::strcmp(toString(FluxSplitter::HLLC), "HLLC") != 0;
+    // the registered recipes keep their ids and names
+    bad += convectionRecipeId(ConvectionScheme::WENO5, FluxSplitter::Rusanov)
+           != TermRecipeId::Weno5Rusanov;
+    bad += std::strcmp(toString(resolveConvectionTermRecipe("weno7Roe").id()), "weno7Roe") != 0;
+    bad += std::strcmp(toString(TermRecipeId::PrimitiveUpwind1), "primitiveUpwind1") != 0;
+    std::printf("%d\n", bad);
+    return bad;
+}
+''')
+    exe = tmp_path / "probe"
+    r = gxx(["-O0", "-w", *ref_includes(_shadow(patched)), str(probe), "-o", str(exe)])
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "0", out.stdout + out.stderr
