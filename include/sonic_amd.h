@@ -158,6 +158,28 @@ int sf_run(sf_grid *grid, int nsteps, double cfl, double max_dt,
 /* Physical time and step count of the grid's clock. */
 int sf_clock(sf_grid *grid, double *time, long long *steps);
 
+/* ---- a whole run that allocates nothing ------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  sf_grid_create + sf_state_upload_device + sf_run +
+ * sf_state_download_device + sf_grid_destroy in one call, on the same
+ * kernels, with no hipMalloc, hipFree or other allocation: the grid's state
+ * buffers, flux accumulator, RK4 stage residuals, clock and fault record
+ * live in one file-scope device array of 64 MiB per GPU (held from the
+ * library's first use of that GPU until the process ends).  A grid needing
+ * more (its *scratch_bytes, if not NULL, is set either way) is rejected
+ * with SF_E_ARG before any work.  soa_device (unique-node SoA, device
+ * memory, 5 N doubles) holds the initial state and receives the result;
+ * dt_seq_device (device memory, nsteps doubles, may be NULL) receives the
+ * dt values; the work is queued on `stream` (a hipStream_t, NULL for the
+ * null stream) and waited for before returning.  Results equal sf_run's bit
+ * for bit, including a fault: SF_E_NUMERIC, with the state at the start of
+ * the faulting step in soa_device.  Every call on one GPU uses that GPU's
+ * one array (each call clears what it needs), so calls that overlap on one
+ * GPU race: call from one host thread per GPU, or serialise the calls. */
+int sf_run_static(const sf_grid_desc *desc, double *soa_device, int nsteps, double cfl,
+                  double max_dt, double *dt_seq_device, void *stream, size_t *scratch_bytes);
+
 /* ---- diagnostics ---------------------------------------------------------- */
 /* Residual R = Jac * sum_d (F_d(i) - F_d(i-1)) of the current state after
  * boundary preparation, on the unique nodes (0 on solver-boundary nodes);

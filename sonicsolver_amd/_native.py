@@ -82,6 +82,8 @@ def _declare(lib):
         "sf_fault_check": [_VP, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_longlong),
                            ctypes.POINTER(ctypes.c_int)],
         "sf_grid_set_march_segment": [_VP, ctypes.c_int],
+        "sf_run_static": [ctypes.POINTER(GridDesc), _VP, ctypes.c_int, ctypes.c_double,
+                          ctypes.c_double, _VP, _VP, ctypes.POINTER(ctypes.c_size_t)],
         "sf_riemann_flux": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                             ctypes.c_double, ctypes.c_int, _P, _P, _P,
                             ctypes.POINTER(ctypes.c_uint32)],
@@ -393,6 +395,18 @@ class Solver:
         _check(load().sf_solver_dt_history(self._h, out.ctypes.data_as(_P), n.value,
                                            ctypes.byref(n)))
         return out
+
+
+def run_static(desc: GridDesc, soa_device: int, nsteps: int, cfl: float, max_dt: float = 1.0e300,
+               dt_seq_device: int = 0, stream: int = 0) -> int:
+    """sf_run_static on device pointers (integers); returns the scratch bytes
+    the grid needs.  Raises SonicError on failure (SF_E_NUMERIC after a
+    fault, with the state at the start of the faulting step in place)."""
+    need = ctypes.c_size_t(0)
+    _check(load().sf_run_static(ctypes.byref(desc), ctypes.c_void_p(soa_device), int(nsteps),
+                                float(cfl), float(max_dt), ctypes.c_void_p(dt_seq_device or None),
+                                ctypes.c_void_p(stream or None), ctypes.byref(need)))
+    return need.value
 
 
 def riemann_flux(flux, axis: int, qL, qR, gamma: float = 1.4, lw: float = 0.0, device: int = 0):

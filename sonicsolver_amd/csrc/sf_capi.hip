@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -59,9 +60,12 @@ struct sf_grid {
     Fault* fault = nullptr;      // device fault record
     unsigned* flags = nullptr;   // &fault->flags
     long long seq = 0;           // sequence number of the next step to begin
-    // (step sequence number, buffer holding the state at its start) of the
-    // steps begun since the fault record was last checked
-    std::vector<std::pair<long long, int>> starts;
+    // The step with sequence number base_seq starts from buffer base_cur;
+    // each step moves the current state by a fixed rotation of the three
+    // buffers (plan_buffers), so the buffer a later step started from
+    // follows from its sequence number.
+    long long base_seq = 0;
+    int base_cur = 0;
     int march_seg = 0;           // nodes per march segment (0: automatic)
     double* dt_hist = nullptr;
     int dt_hist_cap = 0;
@@ -229,6 +233,9 @@ int ensure_hist(sf_grid* g, int n) {
     return 0;
 }
 
+// Buffers the current state moves by in one step (plan_buffers).
+int rotation(int time) { return time == SF_TIME_RK4 ? 2 : 1; }
+
 // Tags of the launches of a step (Fault::tag): the open step's stages and
 // fills, the fill that prepares or commits a step, the CFL scan before it.
 long long stage_tag(const sf_grid* g, int s) { return (g->seq - 1) * 8 + s; }
@@ -252,14 +259,16 @@ int check_flags(sf_grid* g, const char* where, uint32_t* flags_out = nullptr,
     if (step_out) *step_out = h.flags ? c.steps + 1 : 0;
     if (phase_out) *phase_out = h.flags ? (int)(h.tag & 7) : -1;
     if (!h.flags) {
-        g->starts.clear();
+        g->base_seq = g->seq;
+        g->base_cur = g->cur;
         return 0;
     }
     const long long seq = h.tag >> 3;
     const int phase = (int)(h.tag & 7);
-    for (const auto& e : g->starts)
-        if (e.first == seq) g->cur = e.second;
-    g->starts.clear();
+    if (seq >= g->base_seq)
+        g->cur = (int)((g->base_cur + (seq - g->base_seq) * rotation(g->desc.time)) % 3);
+    g->base_seq = g->seq;
+    g->base_cur = g->cur;
     g->step_open = 0;
     g->ghosts_valid = 0;
     g->dtmin_valid = 0;
@@ -359,48 +368,8 @@ void plan_buffers(sf_grid* g) {
     }
 }
 
-int grid_free(sf_grid* g) {
-    if (g->stream) (void)hipStreamSynchronize(g->stream);
-    for (double* p : g->buf) if (p) (void)hipFree(p);
-    for (double* p : g->k) if (p) (void)hipFree(p);
-    if (g->S) (void)hipFree(g->S);
-    if (g->clock) (void)hipFree(g->clock);
-    if (g->fault) (void)hipFree(g->fault);
-    if (g->dt_hist) (void)hipFree(g->dt_hist);
-    if (g->stage_io) (void)hipFree(g->stage_io);
-    for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
-    if (g->stream) (void)hipStreamDestroy(g->stream);
-    delete g;
-    return 0;
-}
-
-}  // namespace
-
-extern "C" {
-
-int sf_last_error(char* buf, size_t len) {
-    if (!buf || len == 0) return SF_E_ARG;
-    std::snprintf(buf, len, "%s", g_last_error.c_str());
-    return 0;
-}
-
-int sf_version(int* major, int* minor) {
-    if (major) *major = 0;
-    if (minor) *minor = 1;
-    return 0;
-}
-
-int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
-    if (!out) return fail(SF_E_ARG, "sf_grid_create: null output handle");
-    *out = nullptr;
-    if (int rc = check_desc(desc)) return rc;
-    int ndev = 0;
-    SF_HIP(hipGetDeviceCount(&ndev));
-    if (desc->device < 0 || desc->device >= ndev)
-        return fail(SF_E_ARG, "sf_grid_create: device ordinal out of range");
-    SF_HIP(hipSetDevice(desc->device));
-    sf_grid* g = new (std::nothrow) sf_grid;
-    if (!g) return fail(SF_E_MEMORY, "sf_grid_create: host allocation failed");
+// Grid constants of a checked description (no device work).
+void set_geometry(sf_grid* g, const sf_grid_desc* desc) {
     g->desc = *desc;
     GridDev& G = g->G;
     for (int a = 0; a < 3; ++a) {
@@ -455,6 +424,52 @@ int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
         const double cof = 0.5 * (c + c);
         G.area[a] = std::sqrt(cof * cof + 0.0 * 0.0 + 0.0 * 0.0);
     }
+}
+
+int grid_free(sf_grid* g) {
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (double* p : g->buf) if (p) (void)hipFree(p);
+    for (double* p : g->k) if (p) (void)hipFree(p);
+    if (g->S) (void)hipFree(g->S);
+    if (g->clock) (void)hipFree(g->clock);
+    if (g->fault) (void)hipFree(g->fault);
+    if (g->dt_hist) (void)hipFree(g->dt_hist);
+    if (g->stage_io) (void)hipFree(g->stage_io);
+    for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sf_last_error(char* buf, size_t len) {
+    if (!buf || len == 0) return SF_E_ARG;
+    std::snprintf(buf, len, "%s", g_last_error.c_str());
+    return 0;
+}
+
+int sf_version(int* major, int* minor) {
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    return 0;
+}
+
+int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
+    if (!out) return fail(SF_E_ARG, "sf_grid_create: null output handle");
+    *out = nullptr;
+    if (int rc = check_desc(desc)) return rc;
+    int ndev = 0;
+    SF_HIP(hipGetDeviceCount(&ndev));
+    if (desc->device < 0 || desc->device >= ndev)
+        return fail(SF_E_ARG, "sf_grid_create: device ordinal out of range");
+    SF_HIP(hipSetDevice(desc->device));
+    sf_grid* g = new (std::nothrow) sf_grid;
+    if (!g) return fail(SF_E_MEMORY, "sf_grid_create: host allocation failed");
+    set_geometry(g, desc);
+    const GridDev& G = g->G;
     const size_t state = sizeof(double) * 5 * (size_t)G.P;
     const size_t N = (size_t)desc->n[0] * desc->n[1] * desc->n[2];
     auto alloc = [&](double** p, size_t bytes) -> int {
@@ -530,7 +545,8 @@ static int upload_from(sf_grid* g, const double* dsrc) {
     const long long N = (long long)G.n[0] * G.n[1] * G.n[2];
     const unsigned nb = std::min<unsigned>(blocks_for(N, 256), 65535u);
     k_clear_fault<<<1, 64, 0, g->stream>>>(g->fault);
-    g->starts.clear();
+    g->base_seq = g->seq;
+    g->base_cur = g->cur;
     k_pack<<<nb, 256, 0, g->stream>>>(G, dsrc, g->buf[g->cur], 1);
     // ends without a condition: every state buffer's ghost layers there hold
     // the uploaded end nodes from now on
@@ -638,7 +654,6 @@ int sf_step_begin(sf_grid* g) {
     if (!g) return fail(SF_E_ARG, "sf_step_begin: null grid");
     if (g->step_open) return fail(SF_E_STATE, "sf_step_begin: a step is already open");
     plan_buffers(g);
-    g->starts.push_back({g->seq, g->cur});
     ++g->seq;
     g->step_open = 1;
     g->next_stage = 0;
@@ -677,6 +692,26 @@ int sf_step_commit(sf_grid* g) {
     return 0;
 }
 
+// nsteps whole steps queued on the grid's stream, dt = min(CFL dt, max_dt)
+// each; hist (device memory, may be null) receives the dt values.
+static int queue_steps(sf_grid* g, int nsteps, double cfl, double max_dt, double* hist) {
+    if (int rc = set_limits(g, max_dt, std::numeric_limits<double>::max())) return rc;
+    g->fold_cfl = 1;
+    g->step_cfl = cfl;
+    for (int s = 0; s < nsteps; ++s) {
+        if (int rc = sf_step_prepare(g)) return rc;
+        if (int rc = scan_dt(g, cfl)) return rc;
+        k_finalize_dt<<<1, 64, 0, g->stream>>>(g->clock, hist, s, g->flags);
+        SF_HIP(hipGetLastError());
+        if (int rc = sf_step_begin(g)) return rc;
+        for (int st = 0; st < nstages(g->desc.time); ++st)
+            if (int rc = sf_stage(g, st)) return rc;
+        if (int rc = sf_step_commit(g)) return rc;
+    }
+    g->fold_cfl = 0;
+    return 0;
+}
+
 int sf_run(sf_grid* g, int nsteps, double cfl, double max_dt, double* dt_seq) {
     if (!g) return fail(SF_E_ARG, "sf_run: null grid");
     if (nsteps < 0) return fail(SF_E_ARG, "sf_run: negative step count");
@@ -687,21 +722,7 @@ int sf_run(sf_grid* g, int nsteps, double cfl, double max_dt, double* dt_seq) {
     if (dt_seq && nsteps > 0) {
         if (int rc = ensure_hist(g, nsteps)) return rc;
     }
-    if (int rc = set_limits(g, max_dt, std::numeric_limits<double>::max())) return rc;
-    g->fold_cfl = 1;
-    g->step_cfl = cfl;
-    for (int s = 0; s < nsteps; ++s) {
-        if (int rc = sf_step_prepare(g)) return rc;
-        if (int rc = scan_dt(g, cfl)) return rc;
-        k_finalize_dt<<<1, 64, 0, g->stream>>>(g->clock, dt_seq ? g->dt_hist : nullptr, s,
-                                               g->flags);
-        SF_HIP(hipGetLastError());
-        if (int rc = sf_step_begin(g)) return rc;
-        for (int st = 0; st < nstages(g->desc.time); ++st)
-            if (int rc = sf_stage(g, st)) return rc;
-        if (int rc = sf_step_commit(g)) return rc;
-    }
-    g->fold_cfl = 0;
+    if (int rc = queue_steps(g, nsteps, cfl, max_dt, dt_seq ? g->dt_hist : nullptr)) return rc;
     if (dt_seq && nsteps > 0) {
         SF_HIP(hipMemcpyAsync(dt_seq, g->dt_hist, sizeof(double) * (size_t)nsteps,
                               hipMemcpyDeviceToHost, g->stream));
@@ -811,3 +832,114 @@ int sf_synchronize(sf_grid* g) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// A whole run that allocates nothing (sf_run_static): the grid's buffers,
+// clock and fault record live in one file-scope device array per GPU.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kStaticBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+// Device address of `scratch` on each device (its name is only a host-side
+// stand-in), taken once per device with that device current.
+constexpr int kMaxDevices = 64;
+unsigned char* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+int static_scratch(int device, unsigned char** out) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(SF_E_ARG, "sf_run_static: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SF_E_HIP, std::string("sf_run_static: hipGetSymbolAddress: ") +
+                                      hipGetErrorString(e));
+        g_scratch[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch[device];
+    return 0;
+}
+
+size_t round256(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+__global__ __launch_bounds__(256) void k_zero(double* __restrict__ p, long long n) {
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < n;
+         u += (long long)gridDim.x * blockDim.x)
+        p[u] = 0.0;
+}
+
+__global__ void k_init_clock(Clock* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->dt = 0.0;
+        c->time = 0.0;
+        c->steps = 0;
+        c->dtmin = kInfBits;
+        c->max_dt = std::numeric_limits<double>::max();
+        c->limit = std::numeric_limits<double>::max();
+    }
+}
+
+}  // namespace
+
+extern "C" int sf_run_static(const sf_grid_desc* desc, double* soa_device, int nsteps,
+                             double cfl, double max_dt, double* dt_seq_device, void* stream,
+                             size_t* scratch_bytes) {
+    if (!soa_device) return fail(SF_E_ARG, "sf_run_static: null state");
+    if (nsteps < 0) return fail(SF_E_ARG, "sf_run_static: negative step count");
+    if (!(cfl > 0.0) || !std::isfinite(cfl)) return fail(SF_E_ARG, "sf_run_static: invalid CFL");
+    if (!(max_dt > 0.0)) return fail(SF_E_ARG, "sf_run_static: max_dt must be positive");
+    if (int rc = check_desc(desc)) return rc;
+    int ndev = 0;
+    SF_HIP(hipGetDeviceCount(&ndev));
+    if (desc->device < 0 || desc->device >= ndev)
+        return fail(SF_E_ARG, "sf_run_static: device ordinal out of range");
+    sf_grid g;
+    set_geometry(&g, desc);
+    // scratch layout: clock, fault record, then the state buffers, the flux
+    // accumulator and (RK4) the three stage residuals
+    const size_t state = round256(sizeof(double) * 5 * (size_t)g.G.P);
+    const int nbuf = desc->time == SF_TIME_RK4 ? 7 : 4;
+    const size_t head = round256(sizeof(Clock)) + round256(sizeof(Fault));
+    const size_t need = head + state * nbuf;
+    if (scratch_bytes) *scratch_bytes = need;
+    if (need > kStaticBytes) {
+        char msg[256];
+        std::snprintf(msg, sizeof msg,
+                      "sf_run_static: the grid needs %zu bytes of device scratch, more than "
+                      "the %zu bytes the library holds per device; use sf_grid_create",
+                      need, kStaticBytes);
+        return fail(SF_E_ARG, msg);
+    }
+    SF_HIP(hipSetDevice(desc->device));
+    unsigned char* base = nullptr;
+    if (int rc = static_scratch(desc->device, &base)) return rc;
+    g.stream = static_cast<hipStream_t>(stream);
+    g.clock = reinterpret_cast<Clock*>(base);
+    g.fault = reinterpret_cast<Fault*>(base + round256(sizeof(Clock)));
+    g.flags = &g.fault->flags;
+    double* bufs[7];
+    for (int i = 0; i < nbuf; ++i) bufs[i] = reinterpret_cast<double*>(base + head + state * i);
+    for (int i = 0; i < 3; ++i) g.buf[i] = bufs[i];
+    g.S = bufs[3];
+    if (nbuf == 7)
+        for (int i = 0; i < 3; ++i) g.k[i] = bufs[4 + i];
+    // Every call on this device shares the array and finds the last call's
+    // contents: the clock, the fault record and the buffers start cleared.
+    k_init_clock<<<1, 64, 0, g.stream>>>(g.clock);
+    k_clear_fault<<<1, 64, 0, g.stream>>>(g.fault);
+    const long long words = (long long)(state * nbuf / sizeof(double));
+    k_zero<<<std::min<unsigned>(blocks_for(words, 256), 4096u), 256, 0, g.stream>>>(bufs[0], words);
+    SF_HIP(hipGetLastError());
+    if (int rc = upload_from(&g, soa_device)) return rc;
+    if (int rc = queue_steps(&g, nsteps, cfl, max_dt, nsteps > 0 ? dt_seq_device : nullptr))
+        return rc;
+    const int status = check_flags(&g, "sf_run_static");
+    if (status != 0 && status != SF_E_NUMERIC) return status;
+    if (int rc = download_to(&g, soa_device)) return rc;
+    SF_HIP(hipStreamSynchronize(g.stream));
+    return status;
+}
