@@ -128,7 +128,7 @@ def _rel_err(a, b):
         float(max((d[v].max() / max(np.abs(b[v]).max(), 1e-300)) for v in range(5)))
 
 
-def batched_run(count: int, steps: int, sample: int):
+def batched_run(count: int, steps: int, sample: int, launch: int = 1, shape=None):
     """Independent 1D Riemann problems of uneven length (256..8192 cells,
     mixed WENO5/TENO5 and Rusanov/HLLC, CFL 0.5 each) advanced together by
     the persistent batch kernels for `steps` steps each, against the same GPU
@@ -141,12 +141,20 @@ def batched_run(count: int, steps: int, sample: int):
     picks = list(range(0, count, max(1, count // sample)))[:sample]
     with Batch(cap, count) as b:
         slots = [b.add(make_item(**kw), q, max_steps=steps) for kw, q in probs]
-        log(f"batched: {count} problems admitted ({cells} cells), autotuning")
-        tuning = b.autotune(3)
-        log(f"batched: tuned {tuning}; timing {steps} steps")
+        if shape:
+            b.set_tiles(shape[0], shape[1])
+            b.set_occupancy(shape[2])
+            b.set_schedule(bool(shape[3]))
+            tuning = {"xlen": shape[0], "ylen": shape[1], "blocks_per_cu": shape[2],
+                      "fused": shape[3], "given": True}
+            log(f"batched: {count} problems admitted ({cells} cells), shape {tuning}")
+        else:
+            log(f"batched: {count} problems admitted ({cells} cells), autotuning")
+            tuning = b.autotune(3)
+            log(f"batched: tuned {tuning}; timing {steps} steps")
         t0 = time.perf_counter()
         try:
-            b.step(steps)
+            b.step(steps, graph=launch)
         except SonicError as e:
             if e.code != -3:
                 raise
@@ -185,6 +193,9 @@ def batched_run(count: int, steps: int, sample: int):
         "cells": cells,
         "solved": solved,
         "failed_nonphysical": len(failed),
+        "failed_problems": failed,
+        "oracle_check": _failure_agreement(failed, count, steps),
+        "launch": {0: "direct", 1: "hipGraph replay", 2: "automatic"}[launch],
         "tuning": tuning,
         "plain_single_case": {
             "solves_per_s": round(plain_solves, 2),
@@ -194,6 +205,23 @@ def batched_run(count: int, steps: int, sample: int):
         "speedup_vs_plain": round(solves / plain_solves, 2),
         "max_rel_diff_vs_plain": {"l1": worst[0], "linf": worst[1]},
     }
+
+
+def _failure_agreement(failed, count, steps):
+    """The problems the device found non-physical against the ones the
+    reference's algorithm fails on (the oracle run over every problem of the
+    batched configuration on the CPU, tests/golden/make_batch_failures.py,
+    stored as a fixture): both counts and whether the sets agree."""
+    path = ROOT / "tests" / "golden" / "batch_riemann1d_failures.json"
+    if not path.exists():
+        return None
+    ref = json.loads(path.read_text())
+    if ref.get("count") != count or ref.get("steps") != steps:
+        return None
+    oracle_failed = sorted(int(k) for k in ref["failed"])
+    return {"device_failed": len(failed), "oracle_failed": len(oracle_failed),
+            "same_problems": sorted(failed) == oracle_failed,
+            "fixture": str(path.relative_to(ROOT))}
 
 
 def configs_1d_2d():
@@ -281,7 +309,7 @@ def recipes_run(n: int, steps: int):
                         f"FP64, {steps} timed steps per recipe", "recipes": out}
 
 
-def _serve_pass(reqs, rate, budget_gb, seed, keep):
+def _serve_pass(reqs, rate, budget_gb, seed, keep, launch=2):
     """One pass of the request stream through a fresh server: Poisson
     arrivals at `rate`/s submitted from a client thread; returns the
     server's statistics, the wall time and the results of the `keep`
@@ -295,6 +323,7 @@ def _serve_pass(reqs, rate, budget_gb, seed, keep):
     arrivals = np.cumsum(gaps) - gaps[0]
     tickets = [None] * count
     with Server(int(budget_gb * 2**30), max_items=1024, steps_per_tick=25) as srv:
+        srv.set_launch(launch)
         # warm the kernels and the graph path on one small request
         kw, st, q = min(reqs, key=lambda r: r[0]["n"][0])
         warm = srv.submit(make_item(**kw), q, 2)
@@ -333,7 +362,8 @@ def _serve_pass(reqs, rate, budget_gb, seed, keep):
     return stats, wall, outs, failed
 
 
-def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int = 5005):
+def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int = 5005,
+               launch: int = 2):
     """Served stream of four-quadrant 2D Riemann problems (N log-uniform in
     [64, 1024], 50-500 steps, WENO5/HLLC/SSPRK3) arriving as a Poisson
     process; requests are admitted into the device arena by footprint as
@@ -347,10 +377,11 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
     cells = sum(kw["n"][0] * kw["n"][1] * st for kw, st, _ in reqs)
     picks = list(range(0, count, max(1, count // sample)))[:sample]
     log(f"served: {count} requests at {rate}/s (Poisson), arena {budget_gb} GiB")
-    sat, sat_wall, outs, failed = _serve_pass(reqs, rate, budget_gb, seed + 1, set(picks))
+    sat, sat_wall, outs, failed = _serve_pass(reqs, rate, budget_gb, seed + 1, set(picks),
+                                              launch)
     moderate = round(0.7 * sat["solves_per_s"], 2)
     log(f"served: {sat['solves_per_s']:.1f} solves/s at saturation; latency pass at {moderate}/s")
-    lat, lat_wall, _, _ = _serve_pass(reqs, moderate, budget_gb, seed + 2, set())
+    lat, lat_wall, _, _ = _serve_pass(reqs, moderate, budget_gb, seed + 2, set(), launch)
     picks = [k for k in picks if k in outs]
     plain_s = 0.0
     worst = (0.0, 0.0)
@@ -378,6 +409,7 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
         "mean_ms": round(lat["mean_ms"], 2),
         "completed": sat["completed"],
         "failed_nonphysical": len(failed),
+        "failed_requests": failed,
         "plain_single_case": {
             "solves_per_s": round(plain_solves, 2),
             "sample": f"{len(picks)} of the requests, each on its own grid, back to back",
@@ -414,6 +446,12 @@ def main():
     ap.add_argument("--batched", type=int, default=4096,
                     help="problems in the batched run (0 skips it)")
     ap.add_argument("--batched-steps", type=int, default=300)
+    ap.add_argument("--batched-launch", choices=["graph", "direct"], default="graph",
+                    help="replay one captured hipGraph per step, or launch each step's kernels")
+    ap.add_argument("--batched-shape", default=None,
+                    help="XLEN,YLEN,BLOCKS_PER_CU,FUSED instead of autotuning")
+    ap.add_argument("--served-launch", choices=["auto", "direct"], default="auto",
+                    help="the request loop's launches: graphs where they pay, or direct")
     ap.add_argument("--served", type=int, default=480,
                     help="requests in the served run (0 skips it)")
     ap.add_argument("--served-rate", type=float, default=120.0,
@@ -558,10 +596,14 @@ def main():
         log(f"every convection recipe on {args.recipes}^3")
         line["recipes"] = recipes_run(args.recipes, 3)
     if args.batched:
-        line["batched"] = batched_run(args.batched, args.batched_steps, args.plain_sample)
+        shape = [int(x) for x in args.batched_shape.split(",")] if args.batched_shape else None
+        line["batched"] = batched_run(args.batched, args.batched_steps, args.plain_sample,
+                                      launch=1 if args.batched_launch == "graph" else 0,
+                                      shape=shape)
     if args.served:
         line["served"] = served_run(args.served, args.served_rate, args.served_budget_gb,
-                                    min(args.plain_sample, 24))
+                                    min(args.plain_sample, 24),
+                                    launch=2 if args.served_launch == "auto" else 0)
     if not args.no_cpu_baseline:
         log(f"CPU baseline: reference code on a {args.cpu_sample}^3 sample")
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cpu_steps)

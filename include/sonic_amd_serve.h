@@ -71,6 +71,11 @@ int sf_server_stats(sf_server *server, sf_serve_stats *stats);
  * _schedule); applied before the loop's next pass.  The default is
  * xlen 255, ylen 32, 16 workgroups per CU, one launch per variant. */
 int sf_server_set_shape(sf_server *server, int xlen, int ylen, int blocks_per_cu, int fused);
+/* How each pass launches its steps, as sf_batch_step's use_graph: 0 direct
+ * launches, 1 one replayed hipGraph per step, 2 (the default) a graph when
+ * the pass is long enough to pay for capturing it.  Results do not depend
+ * on it. */
+int sf_server_set_launch(sf_server *server, int use_graph);
 /* Forget completed latencies and counters (the queue is kept). */
 int sf_server_reset_stats(sf_server *server);
 

@@ -170,11 +170,9 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
                 for (int v = 0; v < 5; ++v) stc[s][v] = q[v * B.P + base + (s - 2)];
             double f5[5];
             face_flux<0, SCHEME, FLUX>(stc, B.gamma, it.area0, f5, flags);
+            if (!finite_sum5(f5)) flags |= kFlagFlux;
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                if (!isfinite(f5[v])) flags |= kFlagFlux;
-                sh[v][t] = f5[v];
-            }
+            for (int v = 0; v < 5; ++v) sh[v][t] = f5[v];
         }
         __syncthreads();
         double dl = INFINITY;
@@ -316,9 +314,9 @@ __device__ __forceinline__ void y_tiles(const BatchDev& B, const double* __restr
             double f[5];
             face_flux<1, SCHEME, FLUX>(stc, B.gamma, it.area1, f, flags);
             double d[5];
+            if (!finite_sum5(f)) flags |= kFlagFlux;
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
-                if (!isfinite(f[v])) flags |= kFlagFlux;
                 d[v] = f[v] - fprev[v];
                 fprev[v] = f[v];
             }

@@ -67,6 +67,7 @@ struct sf_grid {
     long long base_seq = 0;
     int base_cur = 0;
     int march_seg = 0;           // nodes per march segment (0: automatic)
+    int cus = 256;               // compute units of the grid's device
     double* dt_hist = nullptr;
     int dt_hist_cap = 0;
     double* stage_io = nullptr;  // unique-node staging for transfers
@@ -122,6 +123,11 @@ int check_desc(const sf_grid_desc* d) {
         return fail(SF_E_ARG, "unknown time recipe");
     const long long N = (long long)d->n[0] * d->n[1] * d->n[2];
     if (N > (1LL << 31)) return fail(SF_E_ARG, "grid exceeds 2^31 nodes");
+    // padded nodes (ghost layers of the widest halo) index in 32 bits; no
+    // grid that fits HBM with its four state buffers comes near the limit
+    long long P = 1;
+    for (int a = 0; a < 3; ++a) P *= d->active[a] ? d->n[a] + 8 : 1;
+    if (P >= (1LL << 31)) return fail(SF_E_ARG, "grid exceeds 2^31 padded nodes");
     // the x sweep numbers faces in 32 bits
     if ((long long)(d->n[0] + 1) * d->n[1] * d->n[2] >= (1LL << 32) - 256)
         return fail(SF_E_ARG, "grid has 2^32 - 2^8 or more x faces");
@@ -198,7 +204,7 @@ int fill_ghosts(sf_grid* g, double* q, int all, long long tag) {
 // (dz + dy) + dx rather than the reference's (dx + dy) + dz, which changes
 // the last bit at most.
 int run_sweeps(sf_grid* g, const double* q, const StageArgs& A) {
-    SweepCall c{g->G, q, g->S, A, g->stream, g->march_seg};
+    SweepCall c{g->G, q, g->S, A, g->stream, g->march_seg, g->cus};
     int axes[3], na = 0;
     for (int a = 2; a >= 0; --a)
         if (g->G.active[a]) axes[na++] = a;
@@ -405,10 +411,11 @@ void set_geometry(sf_grid* g, const sf_grid_desc* desc) {
     G.stride[1] = G.m[0];
     G.stride[2] = (long long)G.m[0] * G.m[1];
     G.P = (long long)G.m[0] * G.m[1] * G.m[2];
+    for (int a = 0; a < 3; ++a) G.ustride[a] = (unsigned)G.stride[a];
     for (int a = 0; a < 3; ++a) {
         const bool img = (G.images >> a) & 1;
-        G.img_lo[a] = img ? G.g[a] : -1;
-        G.img_hi[a] = img ? G.n[a] - G.g[a] : std::numeric_limits<int>::max();
+        G.img_lo[a] = img ? G.g[a] : 0;
+        G.img_span[a] = img ? (unsigned)(G.n[a] - 2 * G.g[a]) : 0xFFFFFFFFu;
         G.img_off[a] = img ? (long long)G.n[a] * G.stride[a] : 0;
     }
     G.gamma = desc->gamma;
@@ -470,6 +477,9 @@ int sf_grid_create(const sf_grid_desc* desc, sf_grid** out) {
     if (!g) return fail(SF_E_MEMORY, "sf_grid_create: host allocation failed");
     set_geometry(g, desc);
     const GridDev& G = g->G;
+    if (hipDeviceGetAttribute(&g->cus, hipDeviceAttributeMultiprocessorCount, desc->device) !=
+            hipSuccess || g->cus <= 0)
+        g->cus = 256;   // MI355X's CU count; only the launch shapes depend on it
     const size_t state = sizeof(double) * 5 * (size_t)G.P;
     const size_t N = (size_t)desc->n[0] * desc->n[1] * desc->n[2];
     auto alloc = [&](double** p, size_t bytes) -> int {
@@ -899,6 +909,9 @@ extern "C" int sf_run_static(const sf_grid_desc* desc, double* soa_device, int n
         return fail(SF_E_ARG, "sf_run_static: device ordinal out of range");
     sf_grid g;
     set_geometry(&g, desc);
+    if (hipDeviceGetAttribute(&g.cus, hipDeviceAttributeMultiprocessorCount, desc->device) !=
+            hipSuccess || g.cus <= 0)
+        g.cus = 256;
     // scratch layout: clock, fault record, then the state buffers, the flux
     // accumulator and (RK4) the three stage residuals
     const size_t state = round256(sizeof(double) * 5 * (size_t)g.G.P);

@@ -37,8 +37,9 @@ struct GridDev {
     int n[3];
     int g[3];
     int m[3];
-    long long P;        // padded nodes per variable
+    long long P;        // padded nodes per variable (below 2^31)
     long long stride[3];
+    unsigned ustride[3];  // the strides in 32 bits, for node_index
     int active[3];
     int bc[3];          // SF_BC_* kind of each axis
     double jac, area[3], gamma;
@@ -47,11 +48,13 @@ struct GridDev {
     // Bit a set: the final sweep writes every node it updates also into the
     // periodic ghost positions of axis a that hold its copies, so no ghost
     // fill is needed for that axis between stages.  A node c of such an axis
-    // has a copy at c + n if c < img_lo[a] and at c - n if c >= img_hi[a]
-    // (img_lo = g, img_hi = n - g, img_off = n * stride); on other axes
-    // img_lo = -1 and img_hi = INT_MAX, so no node has a copy.
+    // has a copy unless g <= c < n - g, i.e. when (unsigned)(c - img_lo[a])
+    // >= img_span[a] (img_lo = g, img_span = n - 2g): at c + n (img_off =
+    // n * stride) if c < g, at c - n otherwise.  On other axes img_lo = 0
+    // and img_span = UINT_MAX, so no node has a copy.
     int images;
-    int img_lo[3], img_hi[3];
+    int img_lo[3];
+    unsigned img_span[3];
     long long img_off[3];
     // Nodes the stage advances along each axis: solved_lo[a] .. solved_hi[a]
     // (a zeroGradient end's node is excluded; see solved_node).
@@ -94,9 +97,11 @@ struct Fault {
 };
 enum { kPhaseFill = 6, kPhaseScan = 7 };
 
+// Padded index of node (i, j, k); the host keeps P below 2^31, so it is
+// formed in 32-bit arithmetic.
 __device__ __forceinline__ long long node_index(const GridDev& G, int i, int j, int k) {
-    return (long long)(k + G.g[2]) * G.stride[2] + (long long)(j + G.g[1]) * G.stride[1]
-         + (i + G.g[0]);
+    return (long long)((unsigned)(k + G.g[2]) * G.ustride[2] + (unsigned)(j + G.g[1]) * G.ustride[1]
+                       + (unsigned)(i + G.g[0]));
 }
 
 // Boundary kind per axis (SF_BC_*): 0 zeroGradient on both ends,
@@ -149,6 +154,14 @@ __device__ __forceinline__ void wave_min_atomic(double v, unsigned long long* ta
     if ((threadIdx.x & 63) == 0 && b != 0x7FF0000000000000ull) atomicMin(target, b);
 }
 
+// Whether all five values are finite, from their sum: a NaN or an infinity
+// among them makes the sum non-finite, and finite values make it finite
+// unless it overflows, which only states already far outside the range of
+// this path's data could do.
+__device__ __forceinline__ bool finite_sum5(const double f[5]) {
+    return isfinite(((f[0] + f[1]) + (f[2] + f[3])) + f[4]);
+}
+
 __device__ __forceinline__ void raise_flags(unsigned f, unsigned* flags) {
     if (f) atomicOr(flags, f);
 }
@@ -170,10 +183,8 @@ __device__ __forceinline__ void write_images(const GridDev& G, double* __restric
     const int c[3] = {i, j, k};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        long long off;
-        if (c[a] < G.img_lo[a]) off = G.img_off[a];
-        else if (c[a] >= G.img_hi[a]) off = -G.img_off[a];
-        else continue;
+        if ((unsigned)(c[a] - G.img_lo[a]) < G.img_span[a]) continue;
+        const long long off = c[a] < G.img_lo[a] ? G.img_off[a] : -G.img_off[a];
 #pragma unroll
         for (int v = 0; v < 5; ++v) out[v * G.P + id + off] = qn[v];
     }
@@ -217,7 +228,7 @@ __device__ __forceinline__ void stage_update_values(const GT& G, const StageArgs
         const double R = G.jac * S[v];
         if (time == 1) {
             const double euler = qc[v] - dt * R;
-            qn[v] = (A.stage == 0 ? 0.0 : A.base * q0v[v]) + A.incr * euler;
+            qn[v] = A.base * q0v[v] + A.incr * euler;   // base = 0 in stage 1
         } else if (time == 0) {
             qn[v] = qc[v] - dt * R;
         } else if (A.stage < 3) {

@@ -329,6 +329,16 @@ __device__ __forceinline__ Closed close_state(const double q[5], double gamma) {
     return s;
 }
 
+// The closure's validity test, as PerfectGasEOS::fromDensityEnergy makes
+// it on each face state the Rusanov flux closes (SF_perfectGasEOS.h:26-33
+// with requireDensity and checked: rho > 0, and p, T, c > 0 and finite;
+// for a perfect gas with rho > 0, T and c are positive and finite exactly
+// when p is): the reference stops on a face whose reconstructed state
+// fails it.
+__device__ __forceinline__ bool closed_ok(const double q[5], const Closed& s) {
+    return q[0] > 0.0 && q[0] < INFINITY && s.p > 0.0 && s.p < INFINITY;
+}
+
 // physicalFlux (SF_rusanovEOS.cpp:14-32) for n = e_ND.
 template <int ND>
 __device__ __forceinline__ void physical_flux(const double q[5], const Closed& s,
@@ -343,8 +353,9 @@ __device__ __forceinline__ void physical_flux(const double q[5], const Closed& s
 // RusanovEOS::faceFlux, SF_rusanovEOS.cpp:47-77.
 template <int ND>
 __device__ __forceinline__ void rusanov(const double qL[5], const double qR[5],
-                                        double gamma, double out[5]) {
+                                        double gamma, double out[5], unsigned& flags) {
     const Closed l = close_state(qL, gamma), r = close_state(qR, gamma);
+    if (!closed_ok(qL, l) || !closed_ok(qR, r)) flags |= kFlagFlux;
     double fL[5], fR[5];
     physical_flux<ND>(qL, l, fL);
     physical_flux<ND>(qR, r, fR);
@@ -360,8 +371,9 @@ __device__ __forceinline__ void rusanov(const double qL[5], const double qR[5],
 // tangential velocity of each side carried unchanged, flux eq. (10.38).
 template <int ND>
 __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
-                                     double gamma, double out[5]) {
+                                     double gamma, double out[5], unsigned& flags) {
     const Closed l = close_state(qL, gamma), r = close_state(qR, gamma);
+    if (!closed_ok(qL, l) || !closed_ok(qR, r)) flags |= kFlagFlux;
     const double unL = l.u[ND], unR = r.u[ND];
     const double sL = fsqrt(qL[0]), sR = fsqrt(qR[0]);
     const double inv = frcp(sL + sR);
@@ -654,12 +666,12 @@ __device__ __forceinline__ void riemann_flux(const double qL[5], const double qR
                                              double gamma, double lw, double out[5],
                                              unsigned& flags) {
     constexpr int nd = Basis<AXIS>::nd;
-    if constexpr (FLUX == 1) hllc<nd>(qL, qR, gamma, out);
+    if constexpr (FLUX == 1) hllc<nd>(qL, qR, gamma, out, flags);
     else if constexpr (FLUX == 2) steger_warming<nd>(qL, qR, gamma, out, flags);
     else if constexpr (FLUX == 3) lax_friedrichs<nd>(qL, qR, gamma, out, flags);
     else if constexpr (FLUX == 4) roe<AXIS>(qL, qR, gamma, out, flags);
     else if constexpr (FLUX == 5) lax_wendroff<nd>(qL, qR, gamma, lw, out, flags);
-    else rusanov<nd>(qL, qR, gamma, out);
+    else rusanov<nd>(qL, qR, gamma, out, flags);
 }
 
 // Flux::characteristicWENOFaceFlux<NS> (SF_WENO.h:185-245) on a face of

@@ -18,7 +18,30 @@ struct SweepCall {
     StageArgs A;
     hipStream_t stream;
     int seg;           // base nodes per march segment (y and z sweeps)
+    int cus;           // compute units of the device
 };
+
+// Grid of the x sweep (a multiple of 8 for the XCD mapping): one block per
+// 128 tiles, at least as many as the device holds at once (compute units x
+// the kernel's resident blocks per unit) and at most 16384.  Blocks beyond
+// the resident ones run in rounds, each paying its start-up again, which
+// costs most where a block gets few tiles: the 2048^2 x sweep ran 0.46 ms
+// with 16384 blocks (4 tiles each) and 0.32 ms with one resident round;
+// 256^3 1.07 and 0.78 ms.  Long runs per block balance worse: 512^3 ran
+// 6.06 ms with 16384 blocks (130 tiles each) and 6.19 ms with one round.
+template <int SCHEME, int FLUX, int TIME>
+unsigned x_blocks(int cus, long long tiles) {
+    static int per_cu = 0;   // one per kernel instantiation
+    if (per_cu == 0) {
+        int n = 0;
+        per_cu = (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                      &n, k_sweep_x<SCHEME, FLUX, TIME>, kXBlock, 0) == hipSuccess && n > 0)
+                     ? n : 8;
+    }
+    const long long resident = (long long)std::max(cus, 8) * per_cu;
+    const long long want = std::min<long long>(std::max<long long>(tiles / 128, resident), 16384);
+    return (unsigned)((std::min<long long>(tiles, want) + 7) & ~7LL);
+}
 
 // The x sweep is always the last sweep of a stage (x is always active and
 // the sweeps run z, y, x), so it is compiled in its final mode only, and the
@@ -31,17 +54,20 @@ int launch_sweep_fm(const SweepCall& c, int axis) {
         if constexpr (MODE != kFinal) {
             return -1;
         } else {
-            // up to 16384 blocks (a multiple of 8: the kernel deals tiles to
-            // XCDs), each looping over tiles of kXBlock faces
+            // one resident block per slot, each looping over tiles of
+            // kXBlock faces
             const long long faces = (long long)(G.n[0] + 1) * G.n[1] * G.n[2];
             const long long tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
-            const unsigned bx = (unsigned)((std::min<long long>(tiles, 16384) + 7) & ~7LL);
+#define SF_LAUNCH_X(T)                                                            \
+    k_sweep_x<SCHEME, FLUX, T><<<x_blocks<SCHEME, FLUX, T>(c.cus, tiles), kXBlock, 0, \
+                                c.stream>>>(G, c.q, c.S, c.A)
             switch (c.A.time) {
-            case 0: k_sweep_x<SCHEME, FLUX, 0><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
-            case 1: k_sweep_x<SCHEME, FLUX, 1><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
-            case 2: k_sweep_x<SCHEME, FLUX, 2><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
-            default: k_sweep_x<SCHEME, FLUX, 3><<<bx, kXBlock, 0, c.stream>>>(G, c.q, c.S, c.A); break;
+            case 0: SF_LAUNCH_X(0); break;
+            case 1: SF_LAUNCH_X(1); break;
+            case 2: SF_LAUNCH_X(2); break;
+            default: SF_LAUNCH_X(3); break;
             }
+#undef SF_LAUNCH_X
             return 0;
         }
     }

@@ -21,21 +21,25 @@ namespace sfamd {
 constexpr int kXBlock = 64;
 
 // The final sweep's operands for node id: its current value, the flux
-// difference the y and z sweeps accumulated (none on a 1D grid) and (where
-// the recipe reads it) its step-start value.
+// difference the y and z sweeps accumulated and its step-start value.  All
+// three are loaded for every lane, without a test: every lane's node index
+// lies in the padded grid (a lane without a node of its own takes node
+// (-1 or 0, j, k)); on a grid without a y axis no sweep writes S, which
+// keeps the zeros it was created with, so its sum is the x difference
+// alone; where the recipe does not read q0 (Euler, the first SSPRK3
+// stage) the values go unused, and there q0 is the stage input itself, so
+// the loads hit the lines the stencil loads bring in.
 template <int TIME>
 __device__ __forceinline__ void stage_operands(const GridDev& G, const StageArgs& A,
                                                const double* __restrict__ q,
                                                const double* __restrict__ S, long long id,
                                                double (&pq)[5], double (&ps)[5],
                                                double (&pq0)[5]) {
-    const bool with_q0 = stage_reads_q0(TIME, A.stage);
-    const bool with_s = G.active[1];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         pq[v] = q[v * G.P + id];
-        ps[v] = with_s ? S[v * G.P + id] : 0.0;
-        pq0[v] = with_q0 ? A.q0[v * G.P + id] : 0.0;
+        ps[v] = S[v * G.P + id];
+        pq0[v] = A.q0[v * G.P + id];
     }
 }
 
@@ -80,29 +84,25 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
         // eight-node stencil leaves no registers to hold them at three waves
         // per SIMD; it loads them after the face.
         const bool upd = t >= 1 && face && a >= 0;
-        double pq[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, ps[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        double pq0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        const long long id = node_index(G, a, j, k);
+        double pq[5], ps[5], pq0[5];
         constexpr bool kEarly = SCHEME != 3;
-        if (kEarly && upd) stage_operands<TIME>(G, A, q, S, node_index(G, a, j, k), pq, ps, pq0);
+        if (kEarly) stage_operands<TIME>(G, A, q, S, id, pq, ps, pq0);
         if (face) {
             constexpr int NS = Stencil<SCHEME>::ns;
-            const long long base = node_index(G, a, j, k);
             double st[NS][5];
 #pragma unroll
             for (int s = 0; s < NS; ++s)
 #pragma unroll
-                for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + base + (s + 1 - NS / 2)];
+                for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + id + (s + 1 - NS / 2)];
             face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags, lw);
-#pragma unroll
-            for (int v = 0; v < 5; ++v)
-                if (!isfinite(f5[v])) flags |= kFlagFlux;
+            if (!finite_sum5(f5)) flags |= kFlagFlux;
         }
         double lower[5];
 #pragma unroll
         for (int v = 0; v < 5; ++v) lower[v] = __shfl_up(f5[v], 1, kXBlock);
         if (upd) {
-            if (!kEarly) stage_operands<TIME>(G, A, q, S, node_index(G, a, j, k), pq, ps, pq0);
-            const long long id = node_index(G, a, j, k);
+            if (!kEarly) stage_operands<TIME>(G, A, q, S, id, pq, ps, pq0);
             double s5[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
@@ -138,8 +138,11 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
 // ---------------------------------------------------------------------------
 constexpr int kMarchBlock = 64;
 
+// WENO5 and WENO3 marches keep four waves per SIMD (the LDS ring allows
+// four: 10 KB per WENO5 wave), which caps them at 128 registers.
 template <int AXIS, int SCHEME, int FLUX, int MODE>
-__global__ __launch_bounds__(kMarchBlock) void k_sweep_march(GridDev G,
+__global__ __launch_bounds__(kMarchBlock)
+void k_sweep_march(GridDev G,
                                                              const double* __restrict__ q,
                                                              double* __restrict__ S, StageArgs A,
                                                              int seg) {

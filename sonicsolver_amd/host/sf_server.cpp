@@ -97,6 +97,7 @@ struct sf_server {
 
     bool shape_pending = false;
     int shape[4] = {255, 32, 16, 0};   // xlen, ylen, blocks per CU, fused
+    int launch = 2;                      // sf_batch_step's use_graph
 
     // loop thread only
     std::deque<RequestPtr> waiting;
@@ -302,7 +303,7 @@ void loop(sf_server* s) {
         for (const RequestPtr& r : s->running) k = std::min(k, r->steps - r->steps_done);
         k = std::max(1LL, k);
         const auto t0 = Clock::now();
-        int rc = sf_batch_step(s->batch, (int)k, 2);
+        int rc = sf_batch_step(s->batch, (int)k, s->launch);
         const double dt = std::chrono::duration<double>(Clock::now() - t0).count();
         {
             std::lock_guard<std::mutex> lk(s->mu);
@@ -546,6 +547,15 @@ int sf_server_set_shape(sf_server* s, int xlen, int ylen, int blocks_per_cu, int
     s->shape[2] = blocks_per_cu;
     s->shape[3] = fused ? 1 : 0;
     s->shape_pending = true;
+    return 0;
+}
+
+int sf_server_set_launch(sf_server* s, int use_graph) {
+    if (!s) return fail(SF_E_ARG, "sf_server_set_launch: null server");
+    if (use_graph < 0 || use_graph > 2)
+        return fail(SF_E_ARG, "sf_server_set_launch: 0 direct, 1 graph, 2 automatic");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->launch = use_graph;
     return 0;
 }
 

@@ -52,6 +52,7 @@ def _lib():
             "sf_server_stats": [_VP, ctypes.POINTER(ServeStats)],
             "sf_server_reset_stats": [_VP],
             "sf_server_set_shape": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+            "sf_server_set_launch": [_VP, ctypes.c_int],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)
@@ -127,6 +128,10 @@ class Server:
 
     def reset_stats(self):
         _check(self._lib.sf_server_reset_stats(self._h))
+
+    def set_launch(self, use_graph: int = 2):
+        """0 direct launches, 1 replayed graphs, 2 automatic (default)."""
+        _check(self._lib.sf_server_set_launch(self._h, int(use_graph)))
 
     def set_shape(self, xlen: int = 255, ylen: int = 32, blocks_per_cu: int = 16,
                   fused: bool = False):

@@ -287,3 +287,38 @@ def test_faulted_problem_is_isolated():
         for k, s in enumerate(slots):
             assert np.array_equal(b.download(s), base[k]), k
             assert b.clock(s) == bclocks[k]
+
+
+def test_batched_configuration_fails_where_the_reference_fails():
+    """All 4096 problems of the batched configuration for its 300 steps: the
+    problems the device finds non-physical are those on which the oracle
+    (bit-identical to the reference's translation units) stops, as the
+    reference's EOS closure, TENO5 cutoff and validateEquationState checks
+    would stop it (the fixture tests/golden/batch_riemann1d_failures.json,
+    written by tests/golden/make_batch_failures.py over every problem on the
+    CPU); each of them is counted up to the step before its fault, as the
+    oracle records."""
+    import json
+    from pathlib import Path
+    ref = json.loads((Path(__file__).parent / "golden" /
+                      "batch_riemann1d_failures.json").read_text())
+    probs = [(kw, q) for _, kw, q in wl.batch_riemann_1d(ref["count"], seed=ref["seed"])]
+    cap = sum(item_nodes(make_item(**kw)) for kw, _ in probs) + 64
+    print(f"batched configuration: {len(probs)} problems, {ref['steps']} steps", flush=True)
+    with Batch(cap, len(probs)) as b:
+        b.set_tiles(255, 32)
+        b.set_occupancy(16)
+        b.set_schedule(False)
+        slots = [b.add(make_item(**kw), q, max_steps=ref["steps"]) for kw, q in probs]
+        with pytest.raises(SonicError) as e:
+            b.step(ref["steps"])
+        assert e.value.code == -3
+        faults = b.faults()
+        steps = b.steps()
+    failed = sorted(k for k in range(len(probs)) if faults[slots[k]])
+    print(f"device: {len(failed)} failed; oracle: {len(ref['failed'])}", flush=True)
+    assert failed == ref["failed"]
+    done = {r["index"]: r["steps_completed"] for r in ref["details"]}
+    for k in range(len(probs)):
+        expected = done.get(k, ref["steps"])
+        assert steps[slots[k]] == expected, (k, steps[slots[k]], expected)
