@@ -309,7 +309,7 @@ def recipes_run(n: int, steps: int):
                         f"FP64, {steps} timed steps per recipe", "recipes": out}
 
 
-def _serve_pass(reqs, rate, budget_gb, seed, keep, launch=2):
+def _serve_pass(reqs, rate, budget_gb, seed, keep, launch=2, shape=None):
     """One pass of the request stream through a fresh server: Poisson
     arrivals at `rate`/s submitted from a client thread; returns the
     server's statistics, the wall time and the results of the `keep`
@@ -324,6 +324,8 @@ def _serve_pass(reqs, rate, budget_gb, seed, keep, launch=2):
     tickets = [None] * count
     with Server(int(budget_gb * 2**30), max_items=1024, steps_per_tick=25) as srv:
         srv.set_launch(launch)
+        if shape:
+            srv.set_shape(*shape)
         # warm the kernels and the graph path on one small request
         kw, st, q = min(reqs, key=lambda r: r[0]["n"][0])
         warm = srv.submit(make_item(**kw), q, 2)
@@ -363,7 +365,7 @@ def _serve_pass(reqs, rate, budget_gb, seed, keep, launch=2):
 
 
 def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int = 5005,
-               launch: int = 2):
+               launch: int = 2, shape=None):
     """Served stream of four-quadrant 2D Riemann problems (N log-uniform in
     [64, 1024], 50-500 steps, WENO5/HLLC/SSPRK3) arriving as a Poisson
     process; requests are admitted into the device arena by footprint as
@@ -378,10 +380,10 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
     picks = list(range(0, count, max(1, count // sample)))[:sample]
     log(f"served: {count} requests at {rate}/s (Poisson), arena {budget_gb} GiB")
     sat, sat_wall, outs, failed = _serve_pass(reqs, rate, budget_gb, seed + 1, set(picks),
-                                              launch)
+                                              launch, shape)
     moderate = round(0.7 * sat["solves_per_s"], 2)
     log(f"served: {sat['solves_per_s']:.1f} solves/s at saturation; latency pass at {moderate}/s")
-    lat, lat_wall, _, _ = _serve_pass(reqs, moderate, budget_gb, seed + 2, set(), launch)
+    lat, lat_wall, _, _ = _serve_pass(reqs, moderate, budget_gb, seed + 2, set(), launch, shape)
     picks = [k for k in picks if k in outs]
     plain_s = 0.0
     worst = (0.0, 0.0)
@@ -452,6 +454,9 @@ def main():
                     help="XLEN,YLEN,BLOCKS_PER_CU,FUSED instead of autotuning")
     ap.add_argument("--served-launch", choices=["auto", "direct"], default="auto",
                     help="the request loop's launches: graphs where they pay, or direct")
+    ap.add_argument("--served-shape", default=None,
+                    help="XLEN,YLEN,BLOCKS_PER_CU,FUSED of the request loop's batch "
+                         "(default: the server's own)")
     ap.add_argument("--served", type=int, default=480,
                     help="requests in the served run (0 skips it)")
     ap.add_argument("--served-rate", type=float, default=120.0,
@@ -603,7 +608,9 @@ def main():
     if args.served:
         line["served"] = served_run(args.served, args.served_rate, args.served_budget_gb,
                                     min(args.plain_sample, 24),
-                                    launch=2 if args.served_launch == "auto" else 0)
+                                    launch=2 if args.served_launch == "auto" else 0,
+                                    shape=[int(x) for x in args.served_shape.split(",")]
+                                    if args.served_shape else None)
     if not args.no_cpu_baseline:
         log(f"CPU baseline: reference code on a {args.cpu_sample}^3 sample")
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cpu_steps)

@@ -303,7 +303,8 @@ int sf_batch_memory(const sf_batch *batch, size_t *used_nodes, size_t *free_node
  * effect at the next step; results do not depend on it. */
 int sf_batch_set_tiles(sf_batch *batch, int xlen, int ylen);
 /* Persistent grid size of the sweeps in workgroups per compute unit
- * (1..64).  Results do not depend on it. */
+ * (1..64), or 0 for as many as each sweep kernel keeps resident (the
+ * runtime's occupancy figure).  Results do not depend on it. */
 int sf_batch_set_occupancy(sf_batch *batch, int blocks_per_cu);
 /* Sweep schedule: fused != 0 runs one launch per sweep for every
  * (scheme, flux) variant (workgroup ranges assigned per variant); 0 runs one

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -355,7 +356,9 @@ __attribute__((amdgpu_waves_per_eu((SCHEME == 0) ? 4 : 3, 8))) void k_batch_y(Ba
     y_tiles<SCHEME, FLUX>(B, q, S, t0, t1, blockIdx.x, gridDim.x, win);
 }
 
-// Transmissive fill along AXIS of every live problem (one block each).
+// Transmissive fill along AXIS of every live problem (blockIdx.x = the
+// problem, blockIdx.y = a share of its boundary points, so a large problem's
+// scattered column accesses are spread over several blocks).
 // PHASE 0 copies density and momentum from the inner neighbour into the
 // boundary node and its ghosts (SF_zeroGradient.cpp:88-130); PHASE 1
 // rebuilds energy from the neighbour's pressure
@@ -370,7 +373,8 @@ __global__ __launch_bounds__(256) void k_batch_fill(BatchDev B, double* __restri
     const int nother = AXIS == 0 ? it.n1 : it.n0;
     const int nax = AXIS == 0 ? it.n0 : it.n1;
     const long long step = AXIS == 0 ? 1 : it.m0;
-    for (int t = threadIdx.x; t < 2 * nother; t += blockDim.x) {
+    for (int t = blockIdx.y * blockDim.x + threadIdx.x; t < 2 * nother;
+         t += gridDim.y * blockDim.x) {
         const int side = t / nother, r = t % nother;
         const int bnode = side ? nax - 1 : 0;
         const long long b = AXIS == 0 ? item_index(it, bnode, r) : item_index(it, r, bnode);
@@ -540,6 +544,7 @@ struct sf_batch {
     int nxt = 0, nyt = 0, nactive = 0;
     int xr[5] = {0, 0, 0, 0, 0}, yr[5] = {0, 0, 0, 0, 0};  // tile ranges per variant
     bool any2d = false;
+    int max_side = 1;           // largest n0 or n1 of the admitted problems
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 };
@@ -583,8 +588,12 @@ int rebuild_tables(sf_batch* b) {
     std::vector<int4> xt, yt;
     std::vector<int> act;
     b->any2d = false;
+    b->max_side = 1;
     for (int s = 0; s < b->item_capacity; ++s)
-        if (b->used[s]) act.push_back(s);
+        if (b->used[s]) {
+            act.push_back(s);
+            b->max_side = std::max({b->max_side, b->items[s].n0, b->items[s].n1});
+        }
     std::vector<int> order = act;
     auto variant = [&](int s) { return 2 * b->items[s].scheme + b->items[s].flux; };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -667,41 +676,70 @@ Split split_blocks(const int r[5], unsigned total) {
     return sp;
 }
 
-unsigned persistent_blocks(const sf_batch* b, int tiles) {
-    return (unsigned)std::max(1, std::min(tiles, b->cus * b->blocks_per_cu));
+// Blocks of `kernel` (block size `threads`) a compute unit holds at once,
+// from the runtime's occupancy calculator (cached per kernel).
+template <class Kernel>
+int resident_per_cu(Kernel kernel, int threads) {
+    static std::map<const void*, int> cache;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    const void* key = reinterpret_cast<const void*>(kernel);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1)
+        n = 4;
+    cache[key] = n;
+    return n;
 }
 
-// ghost fill of every live problem: copy x, copy y, energy x, energy y
+// Persistent grid of a sweep over `tiles` tiles: blocks_per_cu blocks per
+// compute unit, or with blocks_per_cu = 0 as many as the device holds at
+// once (`per_cu` = the kernel's resident blocks per unit).
+unsigned persistent_blocks(const sf_batch* b, int tiles, int per_cu = 0) {
+    const int want = b->blocks_per_cu > 0 ? b->blocks_per_cu : std::max(per_cu, 1);
+    return (unsigned)std::max(1, std::min(tiles, b->cus * want));
+}
+
+// ghost fill of every live problem: copy x, copy y, energy x, energy y; up
+// to 8 blocks per problem, one per 512 of its boundary points
 void enqueue_fill(sf_batch* b, double* q) {
     const BatchDev B = view(b);
-    const unsigned n = (unsigned)b->nactive;
-    k_batch_fill<0, 0><<<n, 256, 0, b->stream>>>(B, q);
-    if (b->any2d) k_batch_fill<1, 0><<<n, 256, 0, b->stream>>>(B, q);
-    k_batch_fill<0, 1><<<n, 256, 0, b->stream>>>(B, q);
-    if (b->any2d) k_batch_fill<1, 1><<<n, 256, 0, b->stream>>>(B, q);
+    const dim3 grid((unsigned)b->nactive, (unsigned)std::min(8, std::max(1, b->max_side / 256)));
+    k_batch_fill<0, 0><<<grid, 256, 0, b->stream>>>(B, q);
+    if (b->any2d) k_batch_fill<1, 0><<<grid, 256, 0, b->stream>>>(B, q);
+    k_batch_fill<0, 1><<<grid, 256, 0, b->stream>>>(B, q);
+    if (b->any2d) k_batch_fill<1, 1><<<grid, 256, 0, b->stream>>>(B, q);
 }
 
 void launch_x(sf_batch* b, const double* q, BatchStage st) {
     const BatchDev B = view(b);
-    const unsigned total = persistent_blocks(b, b->nxt);
     if (b->fused) {
-        const Split sp = split_blocks(b->xr, total);
-        const unsigned g = (unsigned)sp.block[4];
+#define SF_XF(BS)                                                                            \
+    do {                                                                                     \
+        const Split sp = split_blocks(                                                       \
+            b->xr, persistent_blocks(b, b->nxt, resident_per_cu(k_batch_x_fused<BS>, BS)));  \
+        k_batch_x_fused<BS><<<(unsigned)sp.block[4], BS, 0, b->stream>>>(B, q, b->S, st, sp); \
+    } while (0)
         switch (b->xbs) {
-            case 64: k_batch_x_fused<64><<<g, 64, 0, b->stream>>>(B, q, b->S, st, sp); break;
-            case 256: k_batch_x_fused<256><<<g, 256, 0, b->stream>>>(B, q, b->S, st, sp); break;
-            default: k_batch_x_fused<128><<<g, 128, 0, b->stream>>>(B, q, b->S, st, sp); break;
+            case 64: SF_XF(64); break;
+            case 256: SF_XF(256); break;
+            default: SF_XF(128); break;
         }
+#undef SF_XF
         return;
     }
     for (int v = 0; v < 4; ++v) {
         const int t0 = b->xr[v], t1 = b->xr[v + 1];
         if (t1 == t0) continue;
-        const unsigned g = persistent_blocks(b, t1 - t0);
-#define SF_XV(BS, SC, FL)                                                                   \
-    do {                                                                                   \
-        if (b->any2d || SC == 1) k_batch_x<BS, SC, FL, true><<<g, BS, 0, b->stream>>>(B, q, b->S, st, t0, t1); \
-        else k_batch_x<BS, SC, FL, false><<<g, BS, 0, b->stream>>>(B, q, b->S, st, t0, t1);  \
+#define SF_XV1(BS, SC, FL, W)                                                              \
+    k_batch_x<BS, SC, FL, W>                                                               \
+        <<<persistent_blocks(b, t1 - t0, resident_per_cu(k_batch_x<BS, SC, FL, W>, BS)),  \
+           BS, 0, b->stream>>>(B, q, b->S, st, t0, t1)
+#define SF_XV(BS, SC, FL)                                   \
+    do {                                                   \
+        if (b->any2d || SC == 1) SF_XV1(BS, SC, FL, true); \
+        else SF_XV1(BS, SC, FL, false);                    \
     } while (0)
 #define SF_XBS(SC, FL)                        \
     switch (b->xbs) {                         \
@@ -717,6 +755,7 @@ void launch_x(sf_batch* b, const double* q, BatchStage st) {
         }
 #undef SF_XBS
 #undef SF_XV
+#undef SF_XV1
     }
 }
 
@@ -724,20 +763,24 @@ void launch_y(sf_batch* b, const double* q) {
     const BatchDev B = view(b);
     if (b->nyt == 0) return;
     if (b->fused) {
-        const Split sp = split_blocks(b->yr, persistent_blocks(b, b->nyt));
+        const Split sp = split_blocks(
+            b->yr, persistent_blocks(b, b->nyt, resident_per_cu(k_batch_y_fused, 64)));
         k_batch_y_fused<<<(unsigned)sp.block[4], 64, 0, b->stream>>>(B, q, b->S, sp);
         return;
     }
     for (int v = 0; v < 4; ++v) {
         const int t0 = b->yr[v], t1 = b->yr[v + 1];
         if (t1 == t0) continue;
-        const unsigned g = persistent_blocks(b, t1 - t0);
+#define SF_YV(SC, FL)                                                                         \
+    k_batch_y<SC, FL><<<persistent_blocks(b, t1 - t0, resident_per_cu(k_batch_y<SC, FL>, 64)), \
+                        64, 0, b->stream>>>(B, q, b->S, t0, t1)
         switch (v) {
-            case 0: k_batch_y<0, 0><<<g, 64, 0, b->stream>>>(B, q, b->S, t0, t1); break;
-            case 1: k_batch_y<0, 1><<<g, 64, 0, b->stream>>>(B, q, b->S, t0, t1); break;
-            case 2: k_batch_y<1, 0><<<g, 64, 0, b->stream>>>(B, q, b->S, t0, t1); break;
-            default: k_batch_y<1, 1><<<g, 64, 0, b->stream>>>(B, q, b->S, t0, t1); break;
+            case 0: SF_YV(0, 0); break;
+            case 1: SF_YV(0, 1); break;
+            case 2: SF_YV(1, 0); break;
+            default: SF_YV(1, 1); break;
         }
+#undef SF_YV
     }
 }
 
@@ -1213,7 +1256,7 @@ extern "C" int sf_batch_autotune(sf_batch* b, int trial_steps, sf_batch_tuning* 
     best.ms_per_step = 1e300;
     int tried = 0;
     const int xlens[3] = {63, 127, 255};
-    const int occs[4] = {2, 4, 8, 16};
+    const int occs[5] = {0, 2, 4, 8, 16};   // 0: as many as stay resident
     std::vector<int> ysegs = b->any2d ? std::vector<int>{16, 32, 64} : std::vector<int>{b->yseg};
     for (int fused = 0; fused < 2 && !rc; ++fused)
         for (int xl : xlens)
@@ -1266,8 +1309,10 @@ extern "C" int sf_batch_set_schedule(sf_batch* b, int fused) {
 
 extern "C" int sf_batch_set_occupancy(sf_batch* b, int blocks_per_cu) {
     if (!b) return bfail(SF_E_ARG, "sf_batch_set_occupancy: null batch");
-    if (blocks_per_cu < 1 || blocks_per_cu > 64)
-        return bfail(SF_E_ARG, "sf_batch_set_occupancy: 1..64 workgroups per compute unit");
+    if (blocks_per_cu < 0 || blocks_per_cu > 64)
+        return bfail(SF_E_ARG,
+                     "sf_batch_set_occupancy: 1..64 workgroups per compute unit, or 0 for "
+                     "as many as stay resident");
     b->blocks_per_cu = blocks_per_cu;
     drop_graph(b);
     return 0;

@@ -539,7 +539,7 @@ int sf_server_stats(sf_server* s, sf_serve_stats* st) {
 int sf_server_set_shape(sf_server* s, int xlen, int ylen, int blocks_per_cu, int fused) {
     if (!s) return fail(SF_E_ARG, "sf_server_set_shape: null server");
     if ((xlen != 63 && xlen != 127 && xlen != 255) || ylen < 4 || ylen > 4096 ||
-        blocks_per_cu < 1 || blocks_per_cu > 64)
+        blocks_per_cu < 0 || blocks_per_cu > 64)
         return fail(SF_E_ARG, "sf_server_set_shape: shape out of range (see sf_batch_set_tiles)");
     std::lock_guard<std::mutex> lk(s->mu);
     s->shape[0] = xlen;
