@@ -50,6 +50,22 @@ int sf_case_describe(const sf_case *c, char *buf, size_t len);
  * receives the final state; steps / end_time may be NULL. */
 int sf_case_run(sf_case *c, const char *output_dir, double *final_soa, int *steps,
                 double *end_time);
+/* Write `count` states (states[w] = N x 5 SoA as sf_case_state lays it out,
+ * stored one after another) as the .vts files of steps[w] / times[w] plus
+ * the <jobName>.pvd index, exactly as sf_case_run writes its output
+ * instants; host memory only, no device work.  Stands in for
+ * ResultWriter::save (infrastructure/io/SF_vtkOutput.cpp:59-67,113-125). */
+int sf_case_write_vts(const sf_case *c, const char *output_dir, int count, const int *steps,
+                      const double *times, const double *states);
+/* The block: node layers per axis (inactive axes included), the scaled
+ * hex vertices (vertex[3 v + a]) and the gas constant. */
+int sf_case_mesh(const sf_case *c, int nodes[3], double vertex[24], double *gas_constant);
+/* The boundary sets the settings of field "rho", "U" or "p" name,
+ * comma-separated in the order the output numbers them (BC_ID). */
+int sf_case_boundary_sets(const sf_case *c, const char *field, char *buf, size_t len);
+/* Membership of every block node (i fastest) in the named node set
+ * (MeshGen's set definitions, SF_meshGen.cpp:514-541). */
+int sf_case_set_mask(const sf_case *c, const char *set, uint8_t *mask);
 
 #ifdef __cplusplus
 }

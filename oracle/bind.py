@@ -3,6 +3,8 @@
   liboracle.so      C restatement of the hot path (oracle/sfcpu.c)
   _ref/libsfref.so  harness over the reference's own translation units
                     (oracle/ref_harness.cpp); present where it was built
+  _ref/libsfvts.so  the reference's structured-grid writer over a reference
+                    Field (oracle/vts_harness.cpp); likewise
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
 this module, and only as the checker.
@@ -19,6 +21,7 @@ HERE = Path(__file__).resolve().parent
 ORACLE_SO = HERE / "liboracle.so"
 REF_SO = HERE / "_ref" / "libsfref.so"
 REF_FMA_SO = HERE / "_ref" / "libsfref_fma.so"
+VTS_SO = HERE / "_ref" / "libsfvts.so"
 
 _P = ctypes.POINTER(ctypes.c_double)
 _U = ctypes.POINTER(ctypes.c_uint)
@@ -120,6 +123,39 @@ def ref_available() -> bool:
 
 def ref_fma_available() -> bool:
     return REF_FMA_SO.exists()
+
+
+def vts_available() -> bool:
+    return VTS_SO.exists()
+
+
+def ref_write_vts(nodes, vertex, active, gamma, gas_constant, states, steps, times,
+                  sets: dict, field_sets: dict, directory: str, job: str, by_step: bool):
+    """The reference's ResultWriter over a reference Field
+    (oracle/vts_harness.cpp): states is a list of (5, N) states on the
+    unique nodes of the active axes, sets maps a node-set name to its mask
+    over the block nodes (i fastest), field_sets maps rho / U / p to the set
+    names of their boundary settings, in order."""
+    lib = ctypes.CDLL(str(VTS_SO))
+    fn = lib.sfref_write_vts
+    fn.restype = ctypes.c_int
+    names = list(sets)
+    masks = np.ascontiguousarray(np.stack([np.asarray(sets[n], dtype=np.uint8) for n in names])
+                                 if names else np.zeros(1, dtype=np.uint8))
+    arr = np.ascontiguousarray(np.stack([np.asarray(s, dtype=np.float64).reshape(5, -1)
+                                         for s in states]))
+    err = ctypes.create_string_buffer(1024)
+    rc = fn((ctypes.c_int * 3)(*nodes), _ptr(np.ascontiguousarray(vertex, dtype=np.float64)),
+            (ctypes.c_int * 3)(*active), ctypes.c_double(gamma), ctypes.c_double(gas_constant),
+            ctypes.c_int(len(steps)), (ctypes.c_int * len(steps))(*steps),
+            _ptr(np.ascontiguousarray(times, dtype=np.float64)), _ptr(arr),
+            ctypes.c_int(len(names)), (ctypes.c_char_p * max(1, len(names)))(*[n.encode() for n in names]),
+            masks.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            ",".join(field_sets.get("rho", [])).encode(), ",".join(field_sets.get("U", [])).encode(),
+            ",".join(field_sets.get("p", [])).encode(), str(directory).encode(), job.encode(),
+            ctypes.c_int(1 if by_step else 0), err, ctypes.c_size_t(len(err)))
+    if rc != 0:
+        raise RuntimeError("reference writer: " + err.value.decode())
 
 
 _ref_fma = None

@@ -33,6 +33,11 @@ def _lib():
             "sf_case_run_control": [_VP, ctypes.POINTER(RunControl)],
             "sf_case_describe": [_VP, ctypes.c_char_p, ctypes.c_size_t],
             "sf_case_run": [_VP, ctypes.c_char_p, _P, ctypes.POINTER(ctypes.c_int), _P],
+            "sf_case_write_vts": [_VP, ctypes.c_char_p, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int), _P, _P],
+            "sf_case_mesh": [_VP, ctypes.POINTER(ctypes.c_int), _P, _P],
+            "sf_case_boundary_sets": [_VP, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t],
+            "sf_case_set_mask": [_VP, ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint8)],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)
@@ -97,6 +102,37 @@ class Case:
         buf = ctypes.create_string_buffer(1 << 16)
         _check(self._lib.sf_case_describe(self._h, buf, len(buf)))
         return buf.value.decode()
+
+    def write_vts(self, output_dir: str, states, steps, times):
+        """Write states (each (5, N) as state() returns it) as the output
+        files of the given steps / times, as run() writes them (host only)."""
+        arr = np.ascontiguousarray(np.stack([np.asarray(s, dtype=np.float64).reshape(5, -1)
+                                             for s in states]))
+        st = (ctypes.c_int * len(steps))(*[int(s) for s in steps])
+        tm = np.ascontiguousarray(times, dtype=np.float64)
+        _check(self._lib.sf_case_write_vts(self._h, str(output_dir).encode(), len(steps), st,
+                                           tm.ctypes.data_as(_P), arr.ctypes.data_as(_P)))
+
+    def mesh(self):
+        """(block nodes per axis, (8, 3) scaled vertices, gas constant)."""
+        nodes = (ctypes.c_int * 3)()
+        vert = np.empty(24)
+        r = ctypes.c_double(0.0)
+        _check(self._lib.sf_case_mesh(self._h, nodes, vert.ctypes.data_as(_P), ctypes.byref(r)))
+        return tuple(nodes), vert.reshape(8, 3), r.value
+
+    def boundary_sets(self, field: str) -> list:
+        buf = ctypes.create_string_buffer(4096)
+        _check(self._lib.sf_case_boundary_sets(self._h, field.encode(), buf, len(buf)))
+        text = buf.value.decode()
+        return text.split(",") if text else []
+
+    def set_mask(self, name: str) -> np.ndarray:
+        nodes, _, _ = self.mesh()
+        out = np.zeros(nodes[0] * nodes[1] * nodes[2], dtype=np.uint8)
+        _check(self._lib.sf_case_set_mask(self._h, name.encode(),
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return out
 
     def run(self, output_dir: str | None = None):
         """Run on the device; returns (final state, steps, end time)."""

@@ -610,6 +610,33 @@ void copy_str(const std::string& s, char* buf, size_t len) {
 
 }  // namespace
 
+namespace {
+
+// The output layout of a case: its block, active axes, gas and the sets
+// its rho / U / p boundary settings name.
+sfamd::host::VtsLayout vts_layout(const sf_case* c) {
+    sfamd::host::VtsLayout lay;
+    for (int a = 0; a < 3; ++a) {
+        lay.nodes[a] = c->mesh.nodes[a];
+        lay.active[a] = c->desc.active[a];
+    }
+    std::memcpy(lay.vertex, c->mesh.vertex, sizeof lay.vertex);
+    lay.gamma = c->gamma;
+    lay.gas_constant = c->gas_constant;
+    // the decoded boundary object iterates its sets in sorted order
+    for (auto [f, sets] : c->field_boundary_sets) {
+        std::sort(sets.begin(), sets.end());
+        sets.erase(std::unique(sets.begin(), sets.end()), sets.end());
+        lay.field_sets[f] = sets;
+    }
+    lay.in_set = [mesh = c->mesh](const std::string& s, int i, int j, int k) {
+        return mesh.in_set(s, i, j, k);
+    };
+    return lay;
+}
+
+}  // namespace
+
 extern "C" {
 
 int sf_case_open(const char* case_dir, sf_case** out) {
@@ -720,6 +747,55 @@ int sf_case_describe(const sf_case* c, char* buf, size_t len) {
     return 0;
 }
 
+int sf_case_mesh(const sf_case* c, int nodes[3], double vertex[24], double* gas_constant) {
+    if (!c) return fail(SF_E_ARG, "sf_case_mesh: null case");
+    for (int a = 0; a < 3; ++a)
+        if (nodes) nodes[a] = c->mesh.nodes[a];
+    if (vertex)
+        for (int v = 0; v < 8; ++v)
+            for (int a = 0; a < 3; ++a) vertex[3 * v + a] = c->mesh.vertex[v][a];
+    if (gas_constant) *gas_constant = c->gas_constant;
+    return 0;
+}
+
+int sf_case_boundary_sets(const sf_case* c, const char* field, char* buf, size_t len) {
+    if (!c || !field || !buf || len == 0) return fail(SF_E_ARG, "sf_case_boundary_sets: invalid argument");
+    const sfamd::host::VtsLayout lay = vts_layout(c);
+    std::string out;
+    auto it = lay.field_sets.find(field);
+    if (it != lay.field_sets.end())
+        for (const std::string& s : it->second) out += (out.empty() ? "" : ",") + s;
+    if (out.size() + 1 > len) return fail(SF_E_ARG, "sf_case_boundary_sets: buffer too small");
+    std::memcpy(buf, out.c_str(), out.size() + 1);
+    return 0;
+}
+
+int sf_case_set_mask(const sf_case* c, const char* set, uint8_t* mask) {
+    if (!c || !set || !mask) return fail(SF_E_ARG, "sf_case_set_mask: invalid argument");
+    const Mesh& m = c->mesh;
+    size_t n = 0;
+    for (int k = 0; k < m.nodes[2]; ++k)
+        for (int j = 0; j < m.nodes[1]; ++j)
+            for (int i = 0; i < m.nodes[0]; ++i, ++n) mask[n] = m.in_set(set, i, j, k) ? 1 : 0;
+    return 0;
+}
+
+int sf_case_write_vts(const sf_case* c, const char* output_dir, int count, const int* steps,
+                      const double* times, const double* states) {
+    if (!c || !output_dir || count < 1 || !steps || !times || !states)
+        return fail(SF_E_ARG, "sf_case_write_vts: invalid argument");
+    const size_t per = c->state.size();
+    try {
+        sfamd::host::VtsSeries series(output_dir, c->job, vts_layout(c),
+                                      c->control.write_by_step != 0);
+        for (int w = 0; w < count; ++w) series.write(states + per * (size_t)w, steps[w], times[w]);
+        series.finish();
+    } catch (const std::exception& e) {
+        return fail(SF_E_ARG, e.what());
+    }
+    return 0;
+}
+
 int sf_case_run(sf_case* c, const char* output_dir, double* final_soa, int* steps,
                 double* end_time) {
     if (!c) return fail(SF_E_ARG, "sf_case_run: null case");
@@ -739,24 +815,7 @@ int sf_case_run(sf_case* c, const char* output_dir, double* final_soa, int* step
     } out{c, grid, nullptr, std::vector<double>(c->state.size()), 0};
     if (!rc && output_dir) {
         try {
-            sfamd::host::VtsLayout lay;
-            for (int a = 0; a < 3; ++a) {
-                lay.nodes[a] = c->mesh.nodes[a];
-                lay.active[a] = c->desc.active[a];
-            }
-            std::memcpy(lay.vertex, c->mesh.vertex, sizeof lay.vertex);
-            lay.gamma = c->gamma;
-            lay.gas_constant = c->gas_constant;
-            // the decoded boundary object iterates its sets in sorted order
-            for (auto [f, sets] : c->field_boundary_sets) {
-                std::sort(sets.begin(), sets.end());
-                sets.erase(std::unique(sets.begin(), sets.end()), sets.end());
-                lay.field_sets[f] = sets;
-            }
-            lay.in_set = [mesh = c->mesh](const std::string& s, int i, int j, int k) {
-                return mesh.in_set(s, i, j, k);
-            };
-            out.series = std::make_unique<sfamd::host::VtsSeries>(output_dir, c->job, lay,
+            out.series = std::make_unique<sfamd::host::VtsSeries>(output_dir, c->job, vts_layout(c),
                                                                   c->control.write_by_step != 0);
             if (c->write_initial) out.series->write(c->state.data(), 0, c->control.start_time);
         } catch (const std::exception& e) {
