@@ -224,6 +224,27 @@ def _failure_agreement(failed, count, steps):
             "fixture": str(path.relative_to(ROOT))}
 
 
+def _served_failure_agreement(failed, count, seed):
+    """The served requests the device found non-physical against the
+    reference's algorithm on the CPU (tests/golden/make_served_failures.py:
+    the oracle over every request with N <= 256 and over each request the
+    device flagged, stored as a fixture): within the checked requests both
+    sets must agree, and every flagged request must fail on the oracle."""
+    path = ROOT / "tests" / "golden" / "served_quadrant_failures.json"
+    if not path.exists():
+        return None
+    ref = json.loads(path.read_text())
+    if ref.get("count") != count or ref.get("seed") != seed:
+        return None
+    checked = set(int(k) for k in ref["checked"])
+    oracle_failed = sorted(int(k) for k in ref["failed"])
+    device_checked = sorted(k for k in failed if k in checked)
+    return {"device_failed": len(failed), "oracle_failed": len(oracle_failed),
+            "checked_requests": len(checked),
+            "same_requests": device_checked == oracle_failed and len(device_checked) == len(failed),
+            "fixture": str(path.relative_to(ROOT))}
+
+
 def configs_1d_2d():
     """The two single-grid configurations besides the 3D headline: the 1000-
     cell shock tube (WENO5/Rusanov/SSPRK3, CFL 0.5, 200 steps) and the 2048^2
@@ -412,6 +433,7 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
         "completed": sat["completed"],
         "failed_nonphysical": len(failed),
         "failed_requests": failed,
+        "oracle_check": _served_failure_agreement(failed, count, seed),
         "plain_single_case": {
             "solves_per_s": round(plain_solves, 2),
             "sample": f"{len(picks)} of the requests, each on its own grid, back to back",

@@ -315,10 +315,11 @@ struct Closed {
     double u[3], p, c, inv_rho;
 };
 
-__device__ __forceinline__ Closed close_state(const double q[5], double gamma) {
+// close_state with the reciprocal of the density given.
+__device__ __forceinline__ Closed close_state_r(const double q[5], double gamma, double inv_rho) {
     Closed s;
     const double rho = q[0];
-    s.inv_rho = frcp(rho);
+    s.inv_rho = inv_rho;
     s.u[0] = q[1] * s.inv_rho;
     s.u[1] = q[2] * s.inv_rho;
     s.u[2] = q[3] * s.inv_rho;
@@ -328,6 +329,11 @@ __device__ __forceinline__ Closed close_state(const double q[5], double gamma) {
     s.c = fsqrt(gamma * s.p * s.inv_rho);
     return s;
 }
+
+__device__ __forceinline__ Closed close_state(const double q[5], double gamma) {
+    return close_state_r(q, gamma, frcp(q[0]));
+}
+
 
 // The closure's validity test, as PerfectGasEOS::fromDensityEnergy makes
 // it on each face state the Rusanov flux closes (SF_perfectGasEOS.h:26-33
@@ -372,10 +378,13 @@ __device__ __forceinline__ void rusanov(const double qL[5], const double qR[5],
 template <int ND>
 __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
                                      double gamma, double out[5], unsigned& flags) {
-    const Closed l = close_state(qL, gamma), r = close_state(qR, gamma);
+    // 1/rho = (rsqrt rho)^2 and sqrt rho = rho rsqrt rho from one refined
+    // reciprocal square root per side
+    const double yL = frsqrt(qL[0]), yR = frsqrt(qR[0]);
+    const Closed l = close_state_r(qL, gamma, yL * yL), r = close_state_r(qR, gamma, yR * yR);
     if (!closed_ok(qL, l) || !closed_ok(qR, r)) flags |= kFlagFlux;
     const double unL = l.u[ND], unR = r.u[ND];
-    const double sL = fsqrt(qL[0]), sR = fsqrt(qR[0]);
+    const double sL = qL[0] * yL, sR = qR[0] * yR;
     const double inv = frcp(sL + sR);
     double ut[3];
 #pragma unroll

@@ -20,6 +20,15 @@ namespace sfamd {
 // ---------------------------------------------------------------------------
 constexpr int kXBlock = 64;
 
+// The value of lane t-1 (lane 0 receives 0): one DPP wave shift per 32-bit
+// half, a VALU move with no LDS round trip.
+__device__ __forceinline__ double lane_shift_up1(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // The final sweep's operands for node id: its current value, the flux
 // difference the y and z sweeps accumulated and its step-start value.  All
 // three are loaded for every lane, without a test: every lane's node index
@@ -100,7 +109,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
         }
         double lower[5];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) lower[v] = __shfl_up(f5[v], 1, kXBlock);
+        for (int v = 0; v < 5; ++v) lower[v] = lane_shift_up1(f5[v]);
         if (upd) {
             if (!kEarly) stage_operands<TIME>(G, A, q, S, id, pq, ps, pq0);
             double s5[5];
@@ -138,8 +147,10 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
 // ---------------------------------------------------------------------------
 constexpr int kMarchBlock = 64;
 
-// WENO5 and WENO3 marches keep four waves per SIMD (the LDS ring allows
-// four: 10 KB per WENO5 wave), which caps them at 128 registers.
+// The WENO5 ring (10 KB per wave) leaves room for four waves per SIMD; the
+// register count decides how many run: the WENO5 / WENO3 marches with the
+// Rusanov or HLLC flux fit 128 registers and run four, the other fluxes'
+// need a few more and run three (holding them to 128 spills).
 template <int AXIS, int SCHEME, int FLUX, int MODE>
 __global__ __launch_bounds__(kMarchBlock)
 void k_sweep_march(GridDev G,

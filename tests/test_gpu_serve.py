@@ -85,3 +85,38 @@ def test_oversized_request_rejected():
         with pytest.raises(SonicError) as e:
             srv.submit(make_item(**kw), q, 10)
         assert e.value.code == -4
+
+
+def test_served_configuration_fails_where_the_reference_fails():
+    """The served configuration's requests (quadrant_requests(480, seed=5005),
+    each for its own step count) that the oracle checked on the CPU (every
+    request with N <= 256, plus every request the device flagged in the
+    bench's served run: the fixture tests/golden/served_quadrant_failures.json,
+    written by tests/golden/make_served_failures.py): the device finds
+    exactly the oracle's failing requests non-physical, each after the
+    oracle's number of completed steps.  Run through one batch, which is
+    what the request loop advances (served results equal batch results bit
+    for bit, test_served_equals_alone_and_oracle)."""
+    import json
+    from pathlib import Path
+    ref = json.loads((Path(__file__).parent / "golden" /
+                      "served_quadrant_failures.json").read_text())
+    checked = set(ref["checked"])
+    reqs = [(i, kw, steps, make) for i, kw, steps, make in
+            wl.quadrant_requests(ref["count"], seed=ref["seed"]) if i in checked]
+    cap = sum(item_nodes(make_item(**kw)) for _, kw, _, _ in reqs) + 64
+    longest = max(steps for _, _, steps, _ in reqs)
+    print(f"served configuration: {len(reqs)} checked requests, up to {longest} steps", flush=True)
+    with Batch(cap, len(reqs)) as b:
+        slots = {i: b.add(make_item(**kw), make(), max_steps=steps) for i, kw, steps, make in reqs}
+        with pytest.raises(SonicError) as e:
+            b.step(longest)
+        assert e.value.code == -3
+        faults = b.faults()
+        done = b.steps()
+    failed = sorted(i for i in slots if faults[slots[i]])
+    print(f"device: {len(failed)} failed; oracle: {len(ref['failed'])}", flush=True)
+    assert failed == ref["failed"]
+    completed = {r["index"]: r["steps_completed"] for r in ref["details"]}
+    for i, kw, steps, _ in reqs:
+        assert done[slots[i]] == completed.get(i, steps), (i, done[slots[i]], completed.get(i, steps))
