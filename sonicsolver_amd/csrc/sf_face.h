@@ -75,22 +75,6 @@ __device__ __forceinline__ double frsqrt(double z) {
 // sqrt(z) as z * rsqrt(z) (z > 0)
 __device__ __forceinline__ double fsqrt(double z) { return z * frsqrt(z); }
 
-// x^6 to within a fraction of an ulp beyond the final rounding: the
-// products are carried with their exact rounding errors (FMA two-product),
-// so the result is the correctly rounded x^6 except within ~2^-100 of a
-// rounding boundary -- the value std::pow(x, 6.0) returns in round-to-nearest
-// except in its own rare last-ulp misses.
-__device__ __forceinline__ double pow6(double x) {
-#pragma clang fp contract(off)
-    const double p2 = x * x;
-    const double e2 = __builtin_fma(x, x, -p2);
-    const double p3 = p2 * x;
-    const double e3 = __builtin_fma(p2, x, -p3) + e2 * x;
-    const double p6 = p3 * p3;
-    const double e6 = __builtin_fma(p3, p3, -p6) + 2.0 * p3 * e3;
-    return p6 + e6;
-}
-
 // The reference's cutoff test RN(g / gs) < CT (CT = 1e-5) without the
 // division.  RN(x) < CT exactly when x < CT - d with d half the gap from CT
 // down to the next double (2^-70 for CT = 1e-5), i.e. when
@@ -110,7 +94,8 @@ __device__ __forceinline__ bool teno_cut(double g, double gs) {
 // candidate values (the same parabola at the same face) and the
 // second-difference part of their smoothness indicators are formed once;
 // the two final divisions share one reciprocal.  Each side's weights are
-// those of weno5 / teno5 above.
+// those of WENO5::weno5_core (SF_WENO5.cpp:42-62) or TENO5::teno5_core
+// (SF_TENO5.cpp:44-92).
 struct PairTerms {
     double qL0, qR0, Sa, Sb;     // candidates (x6): left S0, right S0, shared
     double bL[3], bR[3];         // smoothness indicators
@@ -164,16 +149,32 @@ __device__ __forceinline__ void weno5_pair(double c0, double c1, double c2, doub
              right);
 }
 
-__device__ __forceinline__ void teno5_weights(const double b[3], double& c0, double& c1,
+// TENO5's stencil selection (SF_TENO5.cpp:60-89): gamma_k = (1 + tau5 /
+// (beta_k + 1e-40))^6 and the cutoff of gamma_k / sum gamma at 1e-5.  The
+// gammas only decide which stencils are kept, so the last bits of the
+// quotients change a result only where a ratio lies within rounding of the
+// cutoff.  Returns false where the sum of the gammas is not finite, which
+// the reference treats as fatal (requireFiniteGammaSum, SF_TENO5.cpp:33-40).
+__device__ __forceinline__ bool teno5_weights(const double b[3], double& c0, double& c1,
                                               double& c2) {
     const double tau5 = fabs(b[0] - b[2]);
-    const double g0 = pow6(1.0 + fdiv(tau5, b[0] + 1.0e-40));
-    const double g1 = pow6(1.0 + fdiv(tau5, b[1] + 1.0e-40));
-    const double g2 = pow6(1.0 + fdiv(tau5, b[2] + 1.0e-40));
+    // tau5 / e_k for the three k from one reciprocal of e0 e1 e2 (each
+    // e_k >= 1e-40, so the product stays far above the subnormal range)
+    const double e0 = b[0] + 1.0e-40, e1 = b[1] + 1.0e-40, e2 = b[2] + 1.0e-40;
+    const double e01 = e0 * e1;
+    const double t = tau5 * frcp(e01 * e2);
+    auto p6 = [](double x) {   // three products: within a few ulps of x^6
+        const double x3 = x * x * x;
+        return x3 * x3;
+    };
+    const double g0 = p6(1.0 + t * (e1 * e2));
+    const double g1 = p6(1.0 + t * (e0 * e2));
+    const double g2 = p6(1.0 + t * e01);
     const double gs = g0 + g1 + g2;
     c0 = teno_cut(g0, gs) ? 0.0 : 1.0;
     c1 = teno_cut(g1, gs) ? 0.0 : 1.0;
     c2 = teno_cut(g2, gs) ? 0.0 : 1.0;
+    return gs < INFINITY;
 }
 
 __device__ __forceinline__ void teno5_pair(double c0, double c1, double c2, double c3,
@@ -181,11 +182,10 @@ __device__ __forceinline__ void teno5_pair(double c0, double c1, double c2, doub
                                            unsigned& flags) {
     const PairTerms t = pair_terms(c0, c1, c2, c3, c4, c5, 0.0);
     double l0, l1, l2, r0, r1, r2;
-    teno5_weights(t.bL, l0, l1, l2);
-    teno5_weights(t.bR, r0, r1, r2);
+    const bool fin = teno5_weights(t.bL, l0, l1, l2) & teno5_weights(t.bR, r0, r1, r2);
     const double aL = 0.1 * l0 + 0.6 * l1 + 0.3 * l2;
     const double aR = 0.1 * r0 + 0.6 * r1 + 0.3 * r2;
-    if (!(aL > 0.0) || !(aR > 0.0)) flags |= kFlagTeno;
+    if (!fin || !(aL > 0.0) || !(aR > 0.0)) flags |= kFlagTeno;
     const double nL = 0.1 * l0 * t.qL0 + 0.6 * l1 * t.Sa + 0.3 * l2 * t.Sb;
     const double nR = 0.1 * r0 * t.qR0 + 0.6 * r1 * t.Sb + 0.3 * r2 * t.Sa;
     div_pair(nL, 6.0 * aL, nR, 6.0 * aR, left, right);

@@ -114,3 +114,22 @@ def test_characteristic_face_flux_matches_the_oracle(scheme, flux, axis):
             assert rc == 0
             scale = max(np.abs(ref).max(), 1e-300)
             assert np.abs(dev[i] - ref).max() <= 1e-11 * scale, (scheme, flux, axis, i)
+
+
+@pytest.mark.parametrize("flux", ["rusanov", "hllc"])
+def test_teno5_non_finite_gamma_sum_is_a_fault(flux):
+    """A stencil whose first three nodes are equal (beta_0 = 0) next to a
+    jump of 1e40 in density: tau5 / (beta_0 + 1e-40) overflows gamma_0, the
+    sum of the gammas is not finite, and the reference stops
+    (requireFiniteGammaSum, SF_TENO5.cpp:33-40).  The oracle reports it and
+    so does the device."""
+    print(f"case teno5 gamma overflow {flux}", flush=True)
+    st = np.empty((1, 6, 5))
+    for s in range(6):
+        rho = 1.0 if s < 3 else 1.0e40
+        st[0, s] = hf.conserved(rho, np.zeros(3), 1.0)
+    cof = np.array([1.0, 0.0, 0.0])
+    rc, _ = ob.char_face_flux(st[0], cof, scheme="teno5", flux=flux)
+    assert rc != 0
+    _, flags = face_flux("teno5", flux, 0, st, area=1.0)
+    assert flags[0] != 0
