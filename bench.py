@@ -408,13 +408,16 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
     picks = [k for k in picks if k in outs]
     plain_s = 0.0
     worst = (0.0, 0.0)
+    rates = []   # (N, cell-updates, seconds) of each plain solve
     for k in picks:
         kw, st, q = reqs[k]
         out, el = _plain_case(kw, q, st)
         plain_s += el
+        rates.append((kw["n"][0], kw["n"][0] * kw["n"][1] * st, el))
         e = _rel_err(outs[k], out)
         worst = (max(worst[0], e[0]), max(worst[1], e[1]))
     plain_solves = len(picks) / plain_s
+    big = sorted(rates)[-max(1, len(rates) // 4):]   # the largest quarter of the sample
     return {
         "workload": f"{count} four-quadrant 2D Riemann requests, N log-uniform [64,1024], "
                     "50-500 steps, WENO5/HLLC/SSPRK3 FP64, Poisson arrivals",
@@ -436,6 +439,11 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
         "oracle_check": _served_failure_agreement(failed, count, seed),
         "plain_single_case": {
             "solves_per_s": round(plain_solves, 2),
+            "mcell_updates_per_s": round(sum(r[1] for r in rates) / plain_s / 1e6, 2),
+            "largest_quarter": {
+                "n": [r[0] for r in big],
+                "mcell_updates_per_s": round(sum(r[1] for r in big) / sum(r[2] for r in big) / 1e6, 2),
+            },
             "sample": f"{len(picks)} of the requests, each on its own grid, back to back",
         },
         "speedup_vs_plain": round(sat["solves_per_s"] / plain_solves, 2),

@@ -358,18 +358,57 @@ __attribute__((amdgpu_waves_per_eu((SCHEME == 0) ? 4 : 3, 8))) void k_batch_y(Ba
 
 // Transmissive fill along AXIS of every live problem (blockIdx.x = the
 // problem, blockIdx.y = a share of its boundary points, so a large problem's
-// scattered column accesses are spread over several blocks).
+// accesses are spread over several blocks).
 // PHASE 0 copies density and momentum from the inner neighbour into the
 // boundary node and its ghosts (SF_zeroGradient.cpp:88-130); PHASE 1
 // rebuilds energy from the neighbour's pressure
 // (SF_thermodynamicClosure.cpp:219-281).  The phases run x then y each, as
 // the grid fill does.
+//
+// Along y a boundary row is contiguous, so a thread per row point writes
+// coalesced.  Along x the boundary node and its ghosts of one row are four
+// adjacent doubles per variable while rows lie a row pitch apart, so there
+// the work is split finer: PHASE 0 gives each (row, variable, node) a
+// thread, PHASE 1 each (row, node) -- every thread of a row forms the
+// boundary node's energy and pressure itself, from the same values, so the
+// ghosts take exactly the pressure the boundary node holds -- and
+// neighbouring lanes write neighbouring doubles.
 template <int AXIS, int PHASE>
 __global__ __launch_bounds__(256) void k_batch_fill(BatchDev B, double* __restrict__ q) {
     const int slot = B.active[blockIdx.x];
     if (!step_live(B, slot)) return;
     const BatchItemDev it = B.items[slot];
     if (AXIS == 1 && it.n1 <= 1) return;
+    if (AXIS == 0) {
+        constexpr int per = PHASE == 0 ? 16 : 4;   // threads per boundary row
+        const int nrow = it.n1;
+        for (int t = blockIdx.y * blockDim.x + threadIdx.x; t < 2 * nrow * per;
+             t += gridDim.y * blockDim.x) {
+            const int pr = t / per, u = t % per;
+            const int side = pr / nrow, r = pr % nrow;
+            const long long b = item_index(it, side ? it.n0 - 1 : 0, r);
+            const long long s = side ? b - 1 : b + 1;
+            const long long dir = side ? 1 : -1;
+            if (PHASE == 0) {
+                const int v = u >> 2, l = u & 3;   // l = 0: the boundary node
+                q[v * B.P + b + l * dir] = q[v * B.P + s];
+            } else {
+                const double ps = pressure_of(q, B.P, s, B.gamma);
+                const double eb = energy_from_pressure(q, B.P, b, B.gamma, ps);
+                if (u == 0) {
+                    q[4 * B.P + b] = eb;
+                    double qb[5] = {q[b], q[B.P + b], q[2 * B.P + b], q[3 * B.P + b], eb};
+                    const double pb = close_state(qb, B.gamma).p;
+                    if (!(ps > 0.0) || !(pb > 0.0)) atomicOr(&B.flags[slot], (unsigned)kFlagState);
+                } else {
+                    double qb[5] = {q[b], q[B.P + b], q[2 * B.P + b], q[3 * B.P + b], eb};
+                    const double pb = close_state(qb, B.gamma).p;
+                    q[4 * B.P + b + u * dir] = energy_from_pressure(q, B.P, b + u * dir, B.gamma, pb);
+                }
+            }
+        }
+        return;
+    }
     const int nother = AXIS == 0 ? it.n1 : it.n0;
     const int nax = AXIS == 0 ? it.n0 : it.n1;
     const long long step = AXIS == 0 ? 1 : it.m0;
@@ -544,7 +583,7 @@ struct sf_batch {
     int nxt = 0, nyt = 0, nactive = 0;
     int xr[5] = {0, 0, 0, 0, 0}, yr[5] = {0, 0, 0, 0, 0};  // tile ranges per variant
     bool any2d = false;
-    int max_side = 1;           // largest n0 or n1 of the admitted problems
+    int max_n0 = 1, max_n1 = 1;  // largest n0 / n1 of the admitted problems
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 };
@@ -588,11 +627,12 @@ int rebuild_tables(sf_batch* b) {
     std::vector<int4> xt, yt;
     std::vector<int> act;
     b->any2d = false;
-    b->max_side = 1;
+    b->max_n0 = b->max_n1 = 1;
     for (int s = 0; s < b->item_capacity; ++s)
         if (b->used[s]) {
             act.push_back(s);
-            b->max_side = std::max({b->max_side, b->items[s].n0, b->items[s].n1});
+            b->max_n0 = std::max(b->max_n0, b->items[s].n0);
+            b->max_n1 = std::max(b->max_n1, b->items[s].n1);
         }
     std::vector<int> order = act;
     auto variant = [&](int s) { return 2 * b->items[s].scheme + b->items[s].flux; };
@@ -701,15 +741,19 @@ unsigned persistent_blocks(const sf_batch* b, int tiles, int per_cu = 0) {
     return (unsigned)std::max(1, std::min(tiles, b->cus * want));
 }
 
-// ghost fill of every live problem: copy x, copy y, energy x, energy y; up
-// to 8 blocks per problem, one per 512 of its boundary points
+// ghost fill of every live problem: copy x, copy y, energy x, energy y;
+// blocks per problem sized so each thread takes about four work items of
+// the largest problem (at most 32)
 void enqueue_fill(sf_batch* b, double* q) {
     const BatchDev B = view(b);
-    const dim3 grid((unsigned)b->nactive, (unsigned)std::min(8, std::max(1, b->max_side / 256)));
-    k_batch_fill<0, 0><<<grid, 256, 0, b->stream>>>(B, q);
-    if (b->any2d) k_batch_fill<1, 0><<<grid, 256, 0, b->stream>>>(B, q);
-    k_batch_fill<0, 1><<<grid, 256, 0, b->stream>>>(B, q);
-    if (b->any2d) k_batch_fill<1, 1><<<grid, 256, 0, b->stream>>>(B, q);
+    auto grid = [&](long long items) {
+        return dim3((unsigned)b->nactive,
+                    (unsigned)std::min<long long>(32, std::max<long long>(1, (items + 1023) / 1024)));
+    };
+    k_batch_fill<0, 0><<<grid(32LL * b->max_n1), 256, 0, b->stream>>>(B, q);
+    if (b->any2d) k_batch_fill<1, 0><<<grid(2LL * b->max_n0), 256, 0, b->stream>>>(B, q);
+    k_batch_fill<0, 1><<<grid(8LL * b->max_n1), 256, 0, b->stream>>>(B, q);
+    if (b->any2d) k_batch_fill<1, 1><<<grid(2LL * b->max_n0), 256, 0, b->stream>>>(B, q);
 }
 
 void launch_x(sf_batch* b, const double* q, BatchStage st) {
