@@ -173,14 +173,16 @@ int fill_ghosts(sf_grid* g, double* q, int all, long long tag) {
     };
     for (int a = 0; a < 3; ++a) {
         if (!zero_gradient(a)) continue;
-        const unsigned nb = blocks_for(plane(a), 256);
+        const int per = a == 0 ? fill_lanes<0>(G, true) : 1;
+        const unsigned nb = blocks_for(plane(a) * per, 256);
         if (a == 0) k_bc_copy4<0><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
         if (a == 1) k_bc_copy4<1><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
         if (a == 2) k_bc_copy4<2><<<nb, 256, 0, g->stream>>>(G, q, g->flags);
     }
     for (int a = 0; a < 3; ++a) {
         if (!zero_gradient(a)) continue;
-        const unsigned nb = blocks_for(plane(a), 256);
+        const int per = a == 0 ? fill_lanes<0>(G, false) : 1;
+        const unsigned nb = blocks_for(plane(a) * per, 256);
         if (a == 0) k_bc_energy<0><<<nb, 256, 0, g->stream>>>(G, q, g->fault, tag);
         if (a == 1) k_bc_energy<1><<<nb, 256, 0, g->stream>>>(G, q, g->fault, tag);
         if (a == 2) k_bc_energy<2><<<nb, 256, 0, g->stream>>>(G, q, g->fault, tag);

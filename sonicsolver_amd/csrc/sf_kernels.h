@@ -100,6 +100,17 @@ __device__ __forceinline__ bool plane_point(const GridDev& G, long long t, int p
     return true;
 }
 
+// Threads per boundary point of a fill along AXIS: along y and z one per
+// point (a point's neighbours on the plane are adjacent in memory); along
+// x, where the boundary node and its ghosts are the adjacent doubles of one
+// row and the plane's points lie a row apart, one per (variable, node) for
+// the copy and one per node for the energy, so that neighbouring lanes
+// write neighbouring doubles.
+template <int AXIS>
+__host__ __device__ __forceinline__ int fill_lanes(const GridDev& G, bool copy) {
+    return AXIS == 0 ? (copy ? 4 : 1) * (G.g[0] + 1) : 1;
+}
+
 // zeroGradient density + momentum (SF_zeroGradient.cpp:88-130) on the
 // zeroGradient ends of the axis; the ghost depth is G.g[AXIS].
 template <int AXIS>
@@ -108,12 +119,18 @@ __global__ __launch_bounds__(256) void k_bc_copy4(GridDev G, double* __restrict_
     if (*flags) return;
     int p[3], side;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (!plane_point<AXIS>(G, t, p, side)) return;
+    const int per = fill_lanes<AXIS>(G, true);
+    if (!plane_point<AXIS>(G, t / per, p, side)) return;
     if (!zg_end(G, AXIS, side)) return;
     const long long step = G.stride[AXIS];
     const long long b = node_index(G, p[0], p[1], p[2]);
     const long long s = side ? b - step : b + step;
     const long long dir = side ? step : -step;
+    if (AXIS == 0) {
+        const int u = (int)(t % per), v = u / (G.g[0] + 1), l = u % (G.g[0] + 1);
+        q[v * G.P + b + l * dir] = q[v * G.P + s];   // l = 0: the boundary node
+        return;
+    }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double val = q[v * G.P + s];
@@ -129,13 +146,30 @@ __global__ __launch_bounds__(256) void k_bc_energy(GridDev G, double* __restrict
     if (fault->flags) return;
     int p[3], side;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (!plane_point<AXIS>(G, t, p, side)) return;
+    const int per = fill_lanes<AXIS>(G, false);
+    if (!plane_point<AXIS>(G, t / per, p, side)) return;
     if (!zg_end(G, AXIS, side)) return;
     const long long step = G.stride[AXIS];
     const long long b = node_index(G, p[0], p[1], p[2]);
     const long long s = side ? b - step : b + step;
     const long long dir = side ? step : -step;
     const double ps = pressure_of(q, G.P, s, G.gamma);
+    if (AXIS == 0) {
+        // each lane of the point forms the boundary node's energy and
+        // pressure from the same values; lane 0 stores it, lane l >= 1
+        // fills ghost l
+        const int u = (int)(t % per);
+        const double eb = energy_from_pressure(q, G.P, b, G.gamma, ps);
+        double qb[5] = {q[b], q[G.P + b], q[2 * G.P + b], q[3 * G.P + b], eb};
+        const double pb = close_state(qb, G.gamma).p;
+        if (u == 0) {
+            q[4 * G.P + b] = eb;
+            if (!(ps > 0.0) || !(pb > 0.0)) raise_fault(kFlagState, &fault->flags, &fault->tag, tag);
+        } else {
+            q[4 * G.P + b + u * dir] = energy_from_pressure(q, G.P, b + u * dir, G.gamma, pb);
+        }
+        return;
+    }
     q[4 * G.P + b] = energy_from_pressure(q, G.P, b, G.gamma, ps);
     const double pb = pressure_of(q, G.P, b, G.gamma);
     if (!(ps > 0.0) || !(pb > 0.0)) raise_fault(kFlagState, &fault->flags, &fault->tag, tag);

@@ -322,3 +322,46 @@ def test_batched_configuration_fails_where_the_reference_fails():
     for k in range(len(probs)):
         expected = done.get(k, ref["steps"])
         assert steps[slots[k]] == expected, (k, steps[slots[k]], expected)
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_mid_step_faults_leave_other_problems_bitwise_unchanged(fused):
+    """Multi-tile problems of the batched configuration that turn
+    non-physical part-way through a step (at a stage's validation or in a
+    face flux's closure, after 2, 2 and 6 complete steps: the fixture's
+    record of the oracle) under 256-lane x tiles, next to healthy 1D and 2D
+    problems: each faulting problem stops at the oracle's step, and every
+    healthy one advances bit for bit as it does without them -- the
+    per-step liveness every stage kernel reads keeps the waves of a block in
+    step when other blocks raise a fault within the same launch."""
+    import json
+    from pathlib import Path
+    ref = json.loads((Path(__file__).parent / "golden" /
+                      "batch_riemann1d_failures.json").read_text())
+    done = {r["index"]: r["steps_completed"] for r in ref["details"]}
+    picks = [2295, 3787, 2770]          # 6688, 4496, 2095 cells
+    bad = {i: (kw, q) for i, kw, q in wl.batch_riemann_1d(ref["count"], seed=ref["seed"])
+           if i in picks}
+    probs = _mixed_problems(count=6)
+    steps = 12
+    base, bclocks = _run(probs, steps, fused=fused, tiles=(255, 32), occupancy=16)
+    cap = (sum(item_nodes(make_item(**kw)) for kw, _ in probs) +
+           sum(item_nodes(make_item(**kw)) for kw, _ in bad.values()) + 64)
+    print(f"mid-step faults, fused={fused}", flush=True)
+    with Batch(cap, len(probs) + len(bad)) as b:
+        b.set_tiles(255, 32)
+        b.set_occupancy(16)
+        b.set_schedule(fused)
+        slots = [b.add(make_item(**kw), q) for kw, q in probs]
+        bslots = {i: b.add(make_item(**kw), q) for i, (kw, q) in bad.items()}
+        with pytest.raises(SonicError) as e:
+            b.step(steps)
+        assert e.value.code == -3
+        faults = b.faults()
+        counts = b.steps()
+        for i, s in bslots.items():
+            assert faults[s] != 0 and counts[s] == done[i], (i, faults[s], counts[s])
+        for k, s in enumerate(slots):
+            assert faults[s] == 0
+            assert np.array_equal(b.download(s), base[k]), k
+            assert b.clock(s) == bclocks[k]
