@@ -761,6 +761,7 @@ int sf_residual(sf_grid* g, double* soa_out) {
     const int scratch = (g->cur + 1) % 3;
     SF_HIP(hipMemsetAsync(g->buf[scratch], 0, sizeof(double) * 5 * (size_t)g->G.P, g->stream));
     StageArgs A{};
+    A.q0 = g->buf[g->cur];   // a valid state, though the residual-only update reads none
     A.qout = g->buf[scratch];
     A.flags = g->flags;
     A.time = 3;  // residual only
