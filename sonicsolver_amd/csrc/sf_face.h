@@ -384,27 +384,25 @@ __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
     const Closed l = close_state_r(qL, gamma, yL * yL), r = close_state_r(qR, gamma, yR * yR);
     if (!closed_ok(qL, l) || !closed_ok(qR, r)) flags |= kFlagFlux;
     const double unL = l.u[ND], unR = r.u[ND];
-    const double sL = qL[0] * yL, sR = qR[0] * yR;
-    const double inv = frcp(sL + sR);
+    // Roe weights sqrt(rho_K) / (sqrt(rho_L) + sqrt(rho_R))
+    const double inv = frcp(qL[0] * yL + qR[0] * yR);
+    const double wL = qL[0] * yL * inv, wR = qR[0] * yR * inv;
     double ut[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) ut[d] = (sL * l.u[d] + sR * r.u[d]) * inv;
+    for (int d = 0; d < 3; ++d) ut[d] = wL * l.u[d] + wR * r.u[d];
     const double HL = (qL[4] + l.p) * l.inv_rho, HR = (qR[4] + r.p) * r.inv_rho;
-    const double Ht = (sL * HL + sR * HR) * inv;
+    const double Ht = wL * HL + wR * HR;
     const double ct = fsqrt((gamma - 1.0) * (Ht - 0.5 * (ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2])));
     const double SL = fmin(unL - l.c, ut[ND] - ct);
     const double SR = fmax(unR + r.c, ut[ND] + ct);
     const double mL = qL[0] * (SL - unL), mR = qR[0] * (SR - unR);
     const double Ss = fdiv(r.p - l.p + unL * mL - unR * mR, mL - mR);
-    if (0.0 <= SL) {
-        physical_flux<ND>(qL, l, out);
-        return;
-    }
-    if (SR <= 0.0) {
-        physical_flux<ND>(qR, r, out);
-        return;
-    }
-    const bool left = 0.0 <= Ss;
+    // One side's physical flux serves all three regions: the left state's
+    // where the fan leaves the face to the right (0 <= S_L) or the contact
+    // does (0 <= S*), the right state's otherwise; the star correction
+    // applies between the outer waves only.
+    const bool supL = 0.0 <= SL, supR = SR <= 0.0;
+    const bool left = supL || (!supR && 0.0 <= Ss);
     double q[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) q[v] = left ? qL[v] : qR[v];
@@ -423,8 +421,9 @@ __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
     for (int d = 0; d < 3; ++d)
         us[1 + d] = scale * ((d == ND) ? s.u[d] + (Ss - un) : s.u[d]);
     us[4] = scale * (q[4] * s.inv_rho + (Ss - un) * Ss) + (Ss - un) * (s.p * rs);
+    const bool star = !supL && !supR;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) out[v] = f[v] + SK * (us[v] - q[v]);
+    for (int v = 0; v < 5; ++v) out[v] = star ? f[v] + SK * (us[v] - q[v]) : f[v];
 }
 
 // Pressure as Numerics::pressure forms it (SF_utility.h:24-29), the form
