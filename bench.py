@@ -39,9 +39,8 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-# Host threads a one-GPU job may use on the GPU box (its CPU share; the box
-# counts every thread against a process guard, whatever the host's own core
-# count is).
+# Host threads a one-GPU job may use on the machines this runs on (its CPU
+# share of a shared multi-GPU host, whatever the host's own core count is).
 HOST_THREAD_SHARE = 16
 
 

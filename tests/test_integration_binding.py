@@ -5,8 +5,8 @@ reference tree and compile: the HLLC recipe registration
 recipe ids and names, parseFluxSplitter, resolveConvectionTermRecipe) and
 the FlowOperations call site (flowOperations.patch).  A probe linked
 against the patched headers resolves the new recipe names as the reference
-resolves its own.  Runs where the reference sources are present (not on
-the GPU box); the patched copies live in a temporary directory."""
+resolves its own.  Runs where the reference sources are present, and is
+skipped elsewhere; the patched copies live in a temporary directory."""
 import shutil
 import subprocess
 from pathlib import Path
