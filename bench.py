@@ -132,7 +132,7 @@ def batched_run(count: int, steps: int, sample: int, launch: int = 1, shape=None
     mixed WENO5/TENO5 and Rusanov/HLLC, CFL 0.5 each) advanced together by
     the persistent batch kernels for `steps` steps each, against the same GPU
     solving them one case at a time."""
-    from sonicsolver_amd import SonicError, workloads as wl
+    from sonicsolver_amd import SonicError, flops, workloads as wl
     from sonicsolver_amd.batch import Batch, item_nodes, make_item
     probs = [(kw, q) for _, kw, q in wl.batch_riemann_1d(count)]
     cells = sum(kw["n"][0] - 1 for kw, _ in probs)
@@ -180,6 +180,10 @@ def batched_run(count: int, steps: int, sample: int, launch: int = 1, shape=None
         worst = (max(worst[0], e[0]), max(worst[1], e[1]))
     solved = count - len(failed)
     solved_cells = cells - sum(probs[k][0]["n"][0] - 1 for k in failed)
+    # algorithmic FP64 work of the solved problems (each recipe's count)
+    fail_set = set(failed)
+    flop = sum((kw["n"][0] - 1) * steps * flops.flops_per_cell_step(1, kw["scheme"], kw["flux"], "ssprk3")
+               for k, (kw, _) in enumerate(probs) if k not in fail_set)
     solves = solved / elapsed
     plain_solves = len(picks) / plain_s
     return {
@@ -188,6 +192,7 @@ def batched_run(count: int, steps: int, sample: int, launch: int = 1, shape=None
                     "each, FP64",
         "solves_per_s": round(solves, 2),
         "mcell_updates_per_s": round(solved_cells * steps / elapsed / 1e6, 2),
+        "fp64_frac": round(flop / elapsed / (FP64_PEAK_TFLOPS * 1e12), 4),
         "seconds": round(elapsed, 4),
         "cells": cells,
         "solved": solved,
@@ -394,7 +399,7 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
     and the ramp-up and the final drain are a small part of the run) and
     measures solves/s from the first submission to the last result; a
     second pass offers 70% of that and measures the latency percentiles."""
-    from sonicsolver_amd import workloads as wl
+    from sonicsolver_amd import flops, workloads as wl
     reqs = [(kw, steps, make()) for _, kw, steps, make in wl.quadrant_requests(count, seed=seed)]
     cells = sum(kw["n"][0] * kw["n"][1] * st for kw, st, _ in reqs)
     picks = list(range(0, count, max(1, count // sample)))[:sample]
@@ -423,6 +428,8 @@ def served_run(count: int, rate: float, budget_gb: float, sample: int, seed: int
         "solves_per_s": round(sat["solves_per_s"], 2),
         "saturation_rate_per_s": rate,
         "mcell_updates_per_s": round(cells / sat_wall / 1e6, 2),
+        "fp64_frac": round(cells * flops.flops_per_cell_step(2, "weno5", "hllc", "ssprk3")
+                           / sat_wall / (FP64_PEAK_TFLOPS * 1e12), 4),
         "gpu_busy_frac": round(sat["busy_s"] / sat_wall, 3),
         "admit_frac": round(sat["admit_s"] / sat_wall, 3),
         "retire_frac": round(sat["retire_s"] / sat_wall, 3),
