@@ -393,16 +393,17 @@ __global__ __launch_bounds__(256) void k_batch_fill(BatchDev B, double* __restri
                 const int v = u >> 2, l = u & 3;   // l = 0: the boundary node
                 q[v * B.P + b + l * dir] = q[v * B.P + s];
             } else {
+                // every thread of the row forms the boundary node's energy
+                // and pressure from the same values; thread 0 stores the
+                // energy, thread l >= 1 fills ghost l
                 const double ps = pressure_of(q, B.P, s, B.gamma);
                 const double eb = energy_from_pressure(q, B.P, b, B.gamma, ps);
+                double qb[5] = {q[b], q[B.P + b], q[2 * B.P + b], q[3 * B.P + b], eb};
+                const double pb = close_state(qb, B.gamma).p;
                 if (u == 0) {
                     q[4 * B.P + b] = eb;
-                    double qb[5] = {q[b], q[B.P + b], q[2 * B.P + b], q[3 * B.P + b], eb};
-                    const double pb = close_state(qb, B.gamma).p;
                     if (!(ps > 0.0) || !(pb > 0.0)) atomicOr(&B.flags[slot], (unsigned)kFlagState);
                 } else {
-                    double qb[5] = {q[b], q[B.P + b], q[2 * B.P + b], q[3 * B.P + b], eb};
-                    const double pb = close_state(qb, B.gamma).p;
                     q[4 * B.P + b + u * dir] = energy_from_pressure(q, B.P, b + u * dir, B.gamma, pb);
                 }
             }
