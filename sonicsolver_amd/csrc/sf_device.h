@@ -142,15 +142,46 @@ __device__ __forceinline__ double local_dt(const GT& G, const double q[5], doubl
     return cfl / spectral;
 }
 
+// A 64-bit value moved across lanes by one DPP control (two 32-bit moves);
+// lanes whose row ROWS leaves out keep their own value.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+    const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+    const unsigned l = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, 0xf, false);
+    const unsigned h = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, 0xf, false);
+    return ((unsigned long long)h << 32) | l;
+}
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ unsigned long long dpp_min_step(unsigned long long b) {
+    const unsigned long long o = dpp_u64<CTRL, ROWS>(b);
+    return o < b ? o : b;
+}
+
+// Minimum of b over the 64 lanes of a wave, all of them active, by DPP
+// moves (no LDS): lane pairs and quads by quad permutations, 8- and
+// 16-lane rows by the half-row and row mirrors, then row 0 into row 1 and
+// row 2 into row 3 by the row-15 broadcast and rows 0-1 into rows 2-3 by
+// the row-31 broadcast, which leaves the minimum in lane 63.
+__device__ __forceinline__ unsigned long long wave_min_bits(unsigned long long b) {
+    b = dpp_min_step<0xB1>(b);         // quad_perm [1,0,3,2]
+    b = dpp_min_step<0x4E>(b);         // quad_perm [2,3,0,1]
+    b = dpp_min_step<0x141>(b);        // row_half_mirror
+    b = dpp_min_step<0x140>(b);        // row_mirror
+    b = dpp_min_step<0x142, 0xa>(b);   // row_bcast:15 into rows 1 and 3
+    b = dpp_min_step<0x143, 0xc>(b);   // row_bcast:31 into rows 2 and 3
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// Wave-wide minimum of a positive double into *target (one atomic per wave;
+// every lane of the wave calls it).  Positive doubles order like their bit
+// patterns; a lane without a value contributes +infinity.
 __device__ __forceinline__ void wave_min_atomic(double v, unsigned long long* target) {
-    // positive doubles order like their bit patterns
-    unsigned long long b = (v > 0.0 && isfinite(v)) ? (unsigned long long)__double_as_longlong(v)
-                                                    : 0x7FF0000000000000ull;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        unsigned long long o = __shfl_xor(b, off, 64);
-        b = o < b ? o : b;
-    }
+    const unsigned long long b = wave_min_bits(
+        (v > 0.0 && isfinite(v)) ? (unsigned long long)__double_as_longlong(v)
+                                 : 0x7FF0000000000000ull);
     if ((threadIdx.x & 63) == 0 && b != 0x7FF0000000000000ull) atomicMin(target, b);
 }
 
@@ -271,16 +302,13 @@ __device__ __forceinline__ void stage_update(const GT& G, const StageArgs& A, lo
     stage_update_values<TIME>(G, A, id, qc, q0v, S, flags, dt_local, qn);
 }
 
-// Block-wide minimum of a positive double into *target (one atomic).
+// Block-wide minimum of a positive double into *target (one atomic; every
+// thread of the block calls it).
 __device__ __forceinline__ void block_min_atomic(double v, unsigned long long* target,
                                                  unsigned long long* lds) {
-    unsigned long long b = (v > 0.0 && isfinite(v)) ? (unsigned long long)__double_as_longlong(v)
-                                                    : 0x7FF0000000000000ull;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        unsigned long long o = __shfl_xor(b, off, 64);
-        b = o < b ? o : b;
-    }
+    unsigned long long b = wave_min_bits(
+        (v > 0.0 && isfinite(v)) ? (unsigned long long)__double_as_longlong(v)
+                                 : 0x7FF0000000000000ull);
     const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     if ((threadIdx.x & 63) == 0) lds[wave] = b;
     __syncthreads();
