@@ -18,7 +18,7 @@
 //                                               evaluated once
 //   the last sweep (x) also applies the stage update, validates the new
 //   state and (on the last stage) folds the CFL scan of the next step into
-//   a wave-level minimum plus one 64-bit atomic per wave.
+//   a wave-level minimum (DPP moves) plus one 64-bit atomic per wave.
 //
 // Sweeps accumulate S = sum_d (F_d(i) - F_d(i-1)) and the update multiplies
 // by Jac once, as Math::localSpatialResidual does
