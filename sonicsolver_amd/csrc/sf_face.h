@@ -46,34 +46,39 @@ template <> struct Basis<2> {
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
 // Reciprocal, quotient and reciprocal square root from the hardware
-// estimate refined by two Newton steps: within about one ulp of the
-// correctly rounded value for the normal, finite, nonzero operands of this
-// path (a zero or non-finite operand yields NaN, which the flux and state
-// checks report).  The IEEE sequences hipcc emits for '/' and sqrt are two
-// to three times longer.
+// estimate (good to about 5e-8 relative) refined by NR Newton steps: with
+// two, within about one ulp of the correctly rounded value for the normal,
+// finite, nonzero operands of this path; with one, within about 10 ulps
+// (reciprocal) and 20 ulps (reciprocal square root).  A zero or non-finite
+// operand yields NaN, which the flux and state checks report.  The IEEE
+// sequences hipcc emits for '/' and sqrt are two to three times longer.
+template <int NR = 2>
 __device__ __forceinline__ double frcp(double b) {
     double r = __builtin_amdgcn_rcp(b);
-    r = fma(fma(-b, r, 1.0), r, r);
-    r = fma(fma(-b, r, 1.0), r, r);
+#pragma unroll
+    for (int i = 0; i < NR; ++i) r = fma(fma(-b, r, 1.0), r, r);
     return r;
 }
 
+template <int NR = 2>
 __device__ __forceinline__ double fdiv(double a, double b) {
-    const double r = frcp(b);
+    const double r = frcp<NR>(b);
     const double q = a * r;
     return fma(fma(-b, q, a), r, q);
 }
 
+template <int NR = 2>
 __device__ __forceinline__ double frsqrt(double z) {
     double y = __builtin_amdgcn_rsq(z);
     const double h = 0.5 * z;
-    y = y * fma(-h * y, y, 1.5);
-    y = y * fma(-h * y, y, 1.5);
+#pragma unroll
+    for (int i = 0; i < NR; ++i) y = y * fma(-h * y, y, 1.5);
     return y;
 }
 
 // sqrt(z) as z * rsqrt(z) (z > 0)
-__device__ __forceinline__ double fsqrt(double z) { return z * frsqrt(z); }
+template <int NR = 2>
+__device__ __forceinline__ double fsqrt(double z) { return z * frsqrt<NR>(z); }
 
 // The reference's cutoff test RN(g / gs) < CT (CT = 1e-5) without the
 // division.  RN(x) < CT exactly when x < CT - d with d half the gap from CT
@@ -125,13 +130,15 @@ __device__ __forceinline__ PairTerms pair_terms(double c0, double c1, double c2,
 
 // a/b and c/d with one reciprocal (both denominators are positive and far
 // from the ends of the exponent range here).
+template <int NR = 2>
 __device__ __forceinline__ void div_pair(double a, double b, double c, double d, double& x,
                                          double& y) {
-    const double r = frcp(b * d);
+    const double r = frcp<NR>(b * d);
     x = a * d * r;
     y = c * b * r;
 }
 
+template <int NR>
 __device__ __forceinline__ void weno5_pair(double c0, double c1, double c2, double c3,
                                            double c4, double c5, double& left,
                                            double& right) {
@@ -145,7 +152,7 @@ __device__ __forceinline__ void weno5_pair(double c0, double c1, double c2, doub
     const double AR = eR1 * eR2, BR = eR0 * eR2, CR = eR0 * eR1;
     const double nL = AL * t.qL0 + (6.0 * BL) * t.Sa + (3.0 * CL) * t.Sb;
     const double nR = AR * t.qR0 + (6.0 * BR) * t.Sb + (3.0 * CR) * t.Sa;
-    div_pair(nL, 6.0 * AL + 36.0 * BL + 18.0 * CL, nR, 6.0 * AR + 36.0 * BR + 18.0 * CR, left,
+    div_pair<NR>(nL, 6.0 * AL + 36.0 * BL + 18.0 * CL, nR, 6.0 * AR + 36.0 * BR + 18.0 * CR, left,
              right);
 }
 
@@ -198,6 +205,7 @@ __device__ __forceinline__ void teno5_pair(double c0, double c1, double c2, doub
 // weights are those of the reference, normalised with one division per
 // side (alpha_k = d_k / e_k multiplied through by e_A e_B) and the two
 // divisions share one reciprocal.
+template <int NR>
 __device__ __forceinline__ void weno3_pair(const double c[4], double& left, double& right) {
     const double A = -c[0] + 5.0 * c[1] + 2.0 * c[2];   // 6 x left S0 = right S1
     const double B = 2.0 * c[1] + 5.0 * c[2] - c[3];    // 6 x left S1 = right S0
@@ -207,7 +215,7 @@ __device__ __forceinline__ void weno3_pair(const double c[4], double& left, doub
     const double d0 = 1.0 / 3.0, d1 = 2.0 / 3.0;
     const double nL = d0 * eB * A + d1 * eA * B, nR = d0 * eA * B + d1 * eB * A;
     const double sL = d0 * eB + d1 * eA, sR = d0 * eA + d1 * eB;
-    div_pair(nL, 6.0 * sL, nR, 6.0 * sR, left, right);
+    div_pair<NR>(nL, 6.0 * sL, nR, 6.0 * sR, left, right);
 }
 
 // Smoothness of one four-node WENO7 candidate over the cell [a, b] of its
@@ -295,17 +303,24 @@ __device__ __forceinline__ void weno7_pair(const double c[8], double& left, doub
 
 // Scheme ids (SF_SCHEME_*) and their stencils (SF_stencil.h:12-17):
 //   0 WENO5 and 1 TENO5 read 6 nodes, 2 WENO3 reads 4, 3 WENO7 reads 8.
-template <int SCHEME> struct Stencil { static constexpr int ns = 6; };
-template <> struct Stencil<2> { static constexpr int ns = 4; };
-template <> struct Stencil<3> { static constexpr int ns = 8; };
+// newton: the Newton steps of every reciprocal and reciprocal square root
+// in a face of the scheme.  One for WENO5 and WENO3, whose weights are
+// smooth functions of their inputs, so a last-bit difference stays a
+// last-bit difference; two for TENO5, whose stencil cutoff turns one into
+// a different stencil, and for WENO7, which is held to the reference's
+// own frozen Sod output.
+template <int SCHEME> struct Stencil { static constexpr int ns = 6, newton = 1; };
+template <> struct Stencil<1> { static constexpr int ns = 6, newton = 2; };
+template <> struct Stencil<2> { static constexpr int ns = 4, newton = 1; };
+template <> struct Stencil<3> { static constexpr int ns = 8, newton = 2; };
 
-template <int SCHEME>
+template <int SCHEME, int NR>
 __device__ __forceinline__ void recon_pair(const double (&ch)[Stencil<SCHEME>::ns], double& left,
                                            double& right, unsigned& flags) {
     if constexpr (SCHEME == 1) teno5_pair(ch[0], ch[1], ch[2], ch[3], ch[4], ch[5], left, right, flags);
-    else if constexpr (SCHEME == 2) weno3_pair(ch, left, right);
+    else if constexpr (SCHEME == 2) weno3_pair<NR>(ch, left, right);
     else if constexpr (SCHEME == 3) weno7_pair(ch, left, right);
-    else weno5_pair(ch[0], ch[1], ch[2], ch[3], ch[4], ch[5], left, right);
+    else weno5_pair<NR>(ch[0], ch[1], ch[2], ch[3], ch[4], ch[5], left, right);
 }
 
 // Closure of SingleFluidStateModel::close + PerfectGasEOS
@@ -316,6 +331,7 @@ struct Closed {
 };
 
 // close_state with the reciprocal of the density given.
+template <int NR = 2>
 __device__ __forceinline__ Closed close_state_r(const double q[5], double gamma, double inv_rho) {
     Closed s;
     const double rho = q[0];
@@ -326,12 +342,13 @@ __device__ __forceinline__ Closed close_state_r(const double q[5], double gamma,
     const double kinetic = 0.5 * rho * (s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]);
     const double e = (q[4] - kinetic) * s.inv_rho;
     s.p = (gamma - 1.0) * rho * e;
-    s.c = fsqrt(gamma * s.p * s.inv_rho);
+    s.c = fsqrt<NR>(gamma * s.p * s.inv_rho);
     return s;
 }
 
+template <int NR = 2>
 __device__ __forceinline__ Closed close_state(const double q[5], double gamma) {
-    return close_state_r(q, gamma, frcp(q[0]));
+    return close_state_r<NR>(q, gamma, frcp<NR>(q[0]));
 }
 
 
@@ -357,10 +374,10 @@ __device__ __forceinline__ void physical_flux(const double q[5], const Closed& s
 }
 
 // RusanovEOS::faceFlux, SF_rusanovEOS.cpp:47-77.
-template <int ND>
+template <int ND, int NR = 2>
 __device__ __forceinline__ void rusanov(const double qL[5], const double qR[5],
                                         double gamma, double out[5], unsigned& flags) {
-    const Closed l = close_state(qL, gamma), r = close_state(qR, gamma);
+    const Closed l = close_state<NR>(qL, gamma), r = close_state<NR>(qR, gamma);
     if (!closed_ok(qL, l) || !closed_ok(qR, r)) flags |= kFlagFlux;
     double fL[5], fR[5];
     physical_flux<ND>(qL, l, fL);
@@ -375,28 +392,28 @@ __device__ __forceinline__ void rusanov(const double qL[5], const double qR[5],
 // Methods for Fluid Dynamics", 3rd ed.: Einfeldt wave speeds from Roe
 // averages, contact speed eq. (10.37), star states eq. (10.39) with the
 // tangential velocity of each side carried unchanged, flux eq. (10.38).
-template <int ND>
+template <int ND, int NR = 2>
 __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
                                      double gamma, double out[5], unsigned& flags) {
     // 1/rho = (rsqrt rho)^2 and sqrt rho = rho rsqrt rho from one refined
     // reciprocal square root per side
-    const double yL = frsqrt(qL[0]), yR = frsqrt(qR[0]);
-    const Closed l = close_state_r(qL, gamma, yL * yL), r = close_state_r(qR, gamma, yR * yR);
+    const double yL = frsqrt<NR>(qL[0]), yR = frsqrt<NR>(qR[0]);
+    const Closed l = close_state_r<NR>(qL, gamma, yL * yL), r = close_state_r<NR>(qR, gamma, yR * yR);
     if (!closed_ok(qL, l) || !closed_ok(qR, r)) flags |= kFlagFlux;
     const double unL = l.u[ND], unR = r.u[ND];
     // Roe weights sqrt(rho_K) / (sqrt(rho_L) + sqrt(rho_R))
-    const double inv = frcp(qL[0] * yL + qR[0] * yR);
+    const double inv = frcp<NR>(qL[0] * yL + qR[0] * yR);
     const double wL = qL[0] * yL * inv, wR = qR[0] * yR * inv;
     double ut[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) ut[d] = wL * l.u[d] + wR * r.u[d];
     const double HL = (qL[4] + l.p) * l.inv_rho, HR = (qR[4] + r.p) * r.inv_rho;
     const double Ht = wL * HL + wR * HR;
-    const double ct = fsqrt((gamma - 1.0) * (Ht - 0.5 * (ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2])));
+    const double ct = fsqrt<NR>((gamma - 1.0) * (Ht - 0.5 * (ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2])));
     const double SL = fmin(unL - l.c, ut[ND] - ct);
     const double SR = fmax(unR + r.c, ut[ND] + ct);
     const double mL = qL[0] * (SL - unL), mR = qR[0] * (SR - unR);
-    const double Ss = fdiv(r.p - l.p + unL * mL - unR * mR, mL - mR);
+    const double Ss = fdiv<NR>(r.p - l.p + unL * mL - unR * mR, mL - mR);
     // One side's physical flux serves all three regions: the left state's
     // where the fan leaves the face to the right (0 <= S_L) or the contact
     // does (0 <= S*), the right state's otherwise; the star correction
@@ -413,7 +430,7 @@ __device__ __forceinline__ void hllc(const double qL[5], const double qR[5],
     double f[5];
     physical_flux<ND>(q, s, f);
     // scale = mK / (SK - Ss); the pressure term p / mK * scale = p / (SK - Ss)
-    const double rs = frcp(SK - Ss);
+    const double rs = frcp<NR>(SK - Ss);
     const double scale = mK * rs;
     double us[5];
     us[0] = scale;
@@ -433,9 +450,10 @@ struct Legacy {
     bool ok;
 };
 
+template <int NR = 2>
 __device__ __forceinline__ Legacy legacy_state(const double q[5], double gamma) {
     Legacy s;
-    s.inv_rho = frcp(q[0]);
+    s.inv_rho = frcp<NR>(q[0]);
     const double ke = 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * s.inv_rho;
     s.p = (gamma - 1.0) * (q[4] - ke);
     s.u[0] = q[1] * s.inv_rho;
@@ -449,12 +467,12 @@ __device__ __forceinline__ Legacy legacy_state(const double q[5], double gamma) 
 // for n = e_ND, with the reference's smoothed split of eigenvalues below
 // 2 eps_sw = 0.02 (Riemann::splitStegerWarming, SF_riemann.cpp:15-67).
 // The three distinct eigenvalues u_n, u_n + a, u_n - a are split once each.
-template <int ND, bool POS>
+template <int ND, bool POS, int NR = 2>
 __device__ __forceinline__ void steger_half(const double q[5], double gamma, double f[5],
                                             unsigned& flags) {
-    const Legacy s = legacy_state(q, gamma);
+    const Legacy s = legacy_state<NR>(q, gamma);
     if (!s.ok) flags |= kFlagFlux;
-    const double a = fsqrt(gamma * s.p * s.inv_rho);
+    const double a = fsqrt<NR>(gamma * s.p * s.inv_rho);
     const double un = s.u[ND];
     const double q2 = s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2];
     auto split = [](double lam) {
@@ -468,20 +486,20 @@ __device__ __forceinline__ void steger_half(const double q[5], double gamma, dou
     const double W1 = 2.0 * (gamma - 1.0) * l0 + l3 + l4;
     const double W2 = l3 - l4;
     const double W3 = l3 + l4;
-    const double common = q[0] * frcp(2.0 * gamma);
+    const double common = q[0] * frcp<NR>(2.0 * gamma);
     f[0] = common * W1;
 #pragma unroll
     for (int d = 0; d < 3; ++d) f[1 + d] = common * ((d == ND) ? s.u[d] * W1 + a * W2 : s.u[d] * W1);
-    f[4] = common * (0.5 * q2 * W1 + a * un * W2 + (a * a) * frcp(gamma - 1.0) * W3);
+    f[4] = common * (0.5 * q2 * W1 + a * un * W2 + (a * a) * frcp<NR>(gamma - 1.0) * W3);
 }
 
 // StegerWarmingFlux::flux (SF_sw.cpp:21-27): F+(qL) + F-(qR).
-template <int ND>
+template <int ND, int NR = 2>
 __device__ __forceinline__ void steger_warming(const double qL[5], const double qR[5],
                                                double gamma, double out[5], unsigned& flags) {
     double fp[5], fm[5];
-    steger_half<ND, true>(qL, gamma, fp, flags);
-    steger_half<ND, false>(qR, gamma, fm, flags);
+    steger_half<ND, true, NR>(qL, gamma, fp, flags);
+    steger_half<ND, false, NR>(qR, gamma, fm, flags);
 #pragma unroll
     for (int v = 0; v < 5; ++v) out[v] = fp[v] + fm[v];
 }
@@ -489,12 +507,12 @@ __device__ __forceinline__ void steger_warming(const double qL[5], const double 
 // One half of the local Lax-Friedrichs splitting, (F(q) +- lambda q) / 2
 // with lambda = |u_n| + a of the state (Riemann::splitLaxFriedrichs,
 // SF_riemann.cpp:125-154).
-template <int ND, bool POS>
+template <int ND, bool POS, int NR = 2>
 __device__ __forceinline__ void laxf_half(const double q[5], double gamma, double f[5],
                                           unsigned& flags) {
-    const Legacy s = legacy_state(q, gamma);
+    const Legacy s = legacy_state<NR>(q, gamma);
     if (!s.ok) flags |= kFlagFlux;
-    const double a = fsqrt(gamma * s.p * s.inv_rho);
+    const double a = fsqrt<NR>(gamma * s.p * s.inv_rho);
     const double un = s.u[ND];
     const double lam = POS ? fabs(un) + a : -(fabs(un) + a);
     double F[5];
@@ -507,12 +525,12 @@ __device__ __forceinline__ void laxf_half(const double q[5], double gamma, doubl
 }
 
 // LaxFriedrichsFlux::flux (SF_lxF.cpp:21-27).
-template <int ND>
+template <int ND, int NR = 2>
 __device__ __forceinline__ void lax_friedrichs(const double qL[5], const double qR[5],
                                                double gamma, double out[5], unsigned& flags) {
     double fp[5], fm[5];
-    laxf_half<ND, true>(qL, gamma, fp, flags);
-    laxf_half<ND, false>(qR, gamma, fm, flags);
+    laxf_half<ND, true, NR>(qL, gamma, fp, flags);
+    laxf_half<ND, false, NR>(qR, gamma, fm, flags);
 #pragma unroll
     for (int v = 0; v < 5; ++v) out[v] = fp[v] + fm[v];
 }
@@ -530,13 +548,13 @@ __device__ __forceinline__ void euler_flux(const double q[5], const Legacy& s, d
 // The eigenvector matrices of Riemann::buildCharacteristicMatrix
 // (SF_riemann.cpp:69-123) at a state q for n = e_AXIS, in the factored
 // form used by the projections below.
-template <int AXIS>
+template <int AXIS, int NR = 2>
 struct CharBasis {
     double u[3], hq2, c, H, un, ul, um, b2, x1;
 
     __device__ __forceinline__ CharBasis(const double q[5], double gamma, unsigned& flags) {
         using B = Basis<AXIS>;
-        const double invR = frcp(q[0]);
+        const double invR = frcp<NR>(q[0]);
         const double ke = 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * invR;
         const double p = (gamma - 1.0) * (q[4] - ke);
         if (!(q[0] > 0.0) || !(p > 0.0)) flags |= kFlagChar;
@@ -545,7 +563,7 @@ struct CharBasis {
         u[2] = q[3] * invR;
         hq2 = 0.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
         const double cz = gamma * p * invR;
-        const double rc = frsqrt(cz);
+        const double rc = frsqrt<NR>(cz);
         c = cz * rc;
         H = (q[4] + p) * invR;
         un = u[B::nd];
@@ -597,16 +615,16 @@ struct CharBasis {
 // rho~ h~, so that basis has sound speed sqrt(gamma) a~ -- and the wave
 // speeds taken in the reference's order (u_n, u_n, u_n, u_n + a~,
 // u_n - a~) against the basis columns (the fourth is its u_n - c wave).
-template <int AXIS>
+template <int AXIS, int NR = 2>
 __device__ __forceinline__ void roe(const double qL[5], const double qR[5], double gamma,
                                     double out[5], unsigned& flags) {
     constexpr int ND = Basis<AXIS>::nd;
     const double gm1 = gamma - 1.0;
-    const Legacy l = legacy_state(qL, gamma), r = legacy_state(qR, gamma);
+    const Legacy l = legacy_state<NR>(qL, gamma), r = legacy_state<NR>(qR, gamma);
     if (!l.ok || !r.ok) flags |= kFlagFlux;
     const double hL = (qL[4] + l.p) * l.inv_rho, hR = (qR[4] + r.p) * r.inv_rho;
-    const double sL = fsqrt(qL[0]), sR = fsqrt(qR[0]);
-    const double inv = frcp(sL + sR);
+    const double sL = fsqrt<NR>(qL[0]), sR = fsqrt<NR>(qR[0]);
+    const double inv = frcp<NR>(sL + sR);
     double ut[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) ut[d] = (sL * l.u[d] + sR * r.u[d]) * inv;
@@ -614,12 +632,12 @@ __device__ __forceinline__ void roe(const double qL[5], const double qR[5], doub
     const double q2t = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
     const double a2 = gm1 * (ht - 0.5 * q2t);
     if (!(a2 > 0.0)) flags |= kFlagFlux;
-    const double at = fsqrt(a2);
+    const double at = fsqrt<NR>(a2);
     const double unt = ut[ND];
     const double delta = 0.1 * at;
     auto fix = [delta](double lam) {
         const double v = fabs(lam);
-        return v < 2.0 * delta ? 0.5 * (v * v * frcp(2.0 * delta) + delta) : v;
+        return v < 2.0 * delta ? 0.5 * (v * v * frcp<NR>(2.0 * delta) + delta) : v;
     };
     const double lam0 = fix(unt), lam3 = fix(unt + at), lam4 = fix(unt - at);
     const double rt = sL * sR;
@@ -627,8 +645,8 @@ __device__ __forceinline__ void roe(const double qL[5], const double qR[5], doub
     qt[0] = rt;
 #pragma unroll
     for (int d = 0; d < 3; ++d) qt[1 + d] = rt * ut[d];
-    qt[4] = rt * (0.5 * q2t + a2 * frcp(gm1));
-    const CharBasis<AXIS> cb(qt, gamma, flags);
+    qt[4] = rt * (0.5 * q2t + a2 * frcp<NR>(gm1));
+    const CharBasis<AXIS, NR> cb(qt, gamma, flags);
     double dq[5], al[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) dq[v] = qR[v] - qL[v];
@@ -650,11 +668,11 @@ __device__ __forceinline__ void roe(const double qL[5], const double qR[5], doub
 // LaxWendroffFlux::flux (SF_lw.cpp:17-40): the physical flux of the
 // Richtmyer half state q* = (qL + qR)/2 - (lambda/2)(F(qR) - F(qL)),
 // lambda = dt / spacing with the step's dt.
-template <int ND>
+template <int ND, int NR = 2>
 __device__ __forceinline__ void lax_wendroff(const double qL[5], const double qR[5],
                                              double gamma, double lambda, double out[5],
                                              unsigned& flags) {
-    const Legacy l = legacy_state(qL, gamma), r = legacy_state(qR, gamma);
+    const Legacy l = legacy_state<NR>(qL, gamma), r = legacy_state<NR>(qR, gamma);
     if (!l.ok || !r.ok || !(lambda > 0.0)) flags |= kFlagFlux;
     double fL[5], fR[5];
     euler_flux<ND>(qL, l, fL);
@@ -662,24 +680,24 @@ __device__ __forceinline__ void lax_wendroff(const double qL[5], const double qR
     double qh[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) qh[v] = 0.5 * (qL[v] + qR[v]) - 0.5 * lambda * (fR[v] - fL[v]);
-    const Legacy h = legacy_state(qh, gamma);
+    const Legacy h = legacy_state<NR>(qh, gamma);
     if (!h.ok) flags |= kFlagFlux;
     euler_flux<ND>(qh, h, out);
 }
 
 // Flux ids (SF_FLUX_*): 0 Rusanov, 1 HLLC, 2 Steger-Warming,
 // 3 Lax-Friedrichs, 4 Roe, 5 Lax-Wendroff (which reads lw = dt / spacing).
-template <int AXIS, int FLUX>
+template <int AXIS, int FLUX, int NR = 2>
 __device__ __forceinline__ void riemann_flux(const double qL[5], const double qR[5],
                                              double gamma, double lw, double out[5],
                                              unsigned& flags) {
     constexpr int nd = Basis<AXIS>::nd;
-    if constexpr (FLUX == 1) hllc<nd>(qL, qR, gamma, out, flags);
-    else if constexpr (FLUX == 2) steger_warming<nd>(qL, qR, gamma, out, flags);
-    else if constexpr (FLUX == 3) lax_friedrichs<nd>(qL, qR, gamma, out, flags);
-    else if constexpr (FLUX == 4) roe<AXIS>(qL, qR, gamma, out, flags);
-    else if constexpr (FLUX == 5) lax_wendroff<nd>(qL, qR, gamma, lw, out, flags);
-    else rusanov<nd>(qL, qR, gamma, out, flags);
+    if constexpr (FLUX == 1) hllc<nd, NR>(qL, qR, gamma, out, flags);
+    else if constexpr (FLUX == 2) steger_warming<nd, NR>(qL, qR, gamma, out, flags);
+    else if constexpr (FLUX == 3) lax_friedrichs<nd, NR>(qL, qR, gamma, out, flags);
+    else if constexpr (FLUX == 4) roe<AXIS, NR>(qL, qR, gamma, out, flags);
+    else if constexpr (FLUX == 5) lax_wendroff<nd, NR>(qL, qR, gamma, lw, out, flags);
+    else rusanov<nd, NR>(qL, qR, gamma, out, flags);
 }
 
 // Flux::characteristicWENOFaceFlux<NS> (SF_WENO.h:185-245) on a face of
@@ -691,11 +709,12 @@ __device__ __forceinline__ void face_flux(const double (&st)[Stencil<SCHEME>::ns
                                           double gamma, double area, double out[5],
                                           unsigned& flags, double lw = 0.0) {
     constexpr int NS = Stencil<SCHEME>::ns;
+    constexpr int NR = Stencil<SCHEME>::newton;
     // face average (SF_WENO.h:207-211) and its characteristic basis
     double qa[5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) qa[v] = 0.5 * (st[NS / 2 - 1][v] + st[NS / 2][v]);
-    const CharBasis<AXIS> cb(qa, gamma, flags);
+    const CharBasis<AXIS, NR> cb(qa, gamma, flags);
     double ch[5][NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -706,12 +725,12 @@ __device__ __forceinline__ void face_flux(const double (&st)[Stencil<SCHEME>::ns
     }
     double wL[5], wR[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) recon_pair<SCHEME>(ch[v], wL[v], wR[v], flags);
+    for (int v = 0; v < 5; ++v) recon_pair<SCHEME, NR>(ch[v], wL[v], wR[v], flags);
     double qL[5], qR[5];
     cb.back(wL, qL);
     cb.back(wR, qR);
     double f[5];
-    riemann_flux<AXIS, FLUX>(qL, qR, gamma, lw, f, flags);
+    riemann_flux<AXIS, FLUX, NR>(qL, qR, gamma, lw, f, flags);
 #pragma unroll
     for (int v = 0; v < 5; ++v) out[v] = f[v] * area;
 }
