@@ -126,7 +126,7 @@ int check_desc(const sf_grid_desc* d) {
     // padded nodes (ghost layers of the widest halo) index in 32 bits; no
     // grid that fits HBM with its four state buffers comes near the limit
     long long P = 1;
-    for (int a = 0; a < 3; ++a) P *= d->active[a] ? d->n[a] + 8 : 1;
+    for (int a = 0; a < 3; ++a) P *= a == 0 ? d->n[0] + 36 : d->active[a] ? d->n[a] + 8 : 1;
     if (P >= (1LL << 31)) return fail(SF_E_ARG, "grid exceeds 2^31 padded nodes");
     // the x sweep numbers faces in 32 bits
     if ((long long)(d->n[0] + 1) * d->n[1] * d->n[2] >= (1LL << 32) - 256)
@@ -387,6 +387,14 @@ void set_geometry(sf_grid* g, const sf_grid_desc* desc) {
         G.g[a] = desc->active[a] ? (desc->scheme == SF_SCHEME_WENO3 ? 2
                                     : desc->scheme == SF_SCHEME_WENO7 ? 4 : 3) : 0;
         G.m[a] = G.n[a] + 2 * G.g[a];
+        if (a == 0) {
+            // rows start on 128-byte lines: node 0 of every row sits 16
+            // doubles into it (its ghosts just before), and the row length
+            // is a multiple of 16, so the marches' 64-column blocks read and
+            // write whole cache lines
+            G.xo = 16;
+            G.m[0] = (G.xo + G.n[0] + G.g[0] + 15) / 16 * 16;
+        }
         G.active[a] = desc->active[a];
         G.bc[a] = desc->bc[a];
         G.d[a] = desc->d[a];

@@ -37,6 +37,7 @@ struct GridDev {
     int n[3];
     int g[3];
     int m[3];
+    int xo;             // position of node i = 0 in its row (a multiple of 16)
     long long P;        // padded nodes per variable (below 2^31)
     long long stride[3];
     unsigned ustride[3];  // the strides in 32 bits, for node_index
@@ -101,7 +102,7 @@ enum { kPhaseFill = 6, kPhaseScan = 7 };
 // formed in 32-bit arithmetic.
 __device__ __forceinline__ long long node_index(const GridDev& G, int i, int j, int k) {
     return (long long)((unsigned)(k + G.g[2]) * G.ustride[2] + (unsigned)(j + G.g[1]) * G.ustride[1]
-                       + (unsigned)(i + G.g[0]));
+                       + (unsigned)(i + G.xo));
 }
 
 // Boundary kind per axis (SF_BC_*): 0 zeroGradient on both ends,
