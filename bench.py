@@ -571,7 +571,8 @@ def main():
     fl = flops.sweep_flops((n, n, n), axis, scheme, flux, final=final, dim=3, time=tscheme,
                            fold_dt=1.0 / 3.0 if final else 0.0)
     achieved_tf = fl / (avg_ms * 1e-3) / 1e12
-    kernel_names = {"sweep_x": "k_sweep_x (x faces + stage update + CFL fold)",
+    kernel_names = {"sweep_x": "the x sweep: k_sweep_xl on grids with many x lines (512^3), "
+                               "else k_sweep_x (x faces + stage update + CFL fold)",
                     "sweep_y": "k_sweep_march<1> (y faces, accumulates S)",
                     "sweep_z": "k_sweep_march<2> (z faces, writes S)"}
     traffic, traffic_src = traffic_measurement(name)

@@ -205,6 +205,12 @@ int sf_fault_check(sf_grid *grid, uint32_t *flags, long long *step, int *phase);
  * else 32 halved down to 8 until there are 4096).  No reference
  * counterpart; results do not depend on it. */
 int sf_grid_set_march_segment(sf_grid *grid, int nodes);
+/* How the x sweep walks the grid: 1 by whole x lines in 64-node tiles,
+ * 0 by tiles of 63 faces over all lines numbered as one sequence, -1 (the
+ * default) by lines where the grid has x lines enough to fill the device
+ * four times over with groups of 16 (512^3), else by flattened tiles.  No
+ * reference counterpart; results do not depend on it. */
+int sf_grid_set_x_walk(sf_grid *grid, int mode);
 /* Wait for the grid's queued work. */
 int sf_synchronize(sf_grid *grid);
 /* The HIP stream (hipStream_t) the grid queues its work on. */

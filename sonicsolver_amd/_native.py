@@ -82,6 +82,7 @@ def _declare(lib):
         "sf_fault_check": [_VP, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_longlong),
                            ctypes.POINTER(ctypes.c_int)],
         "sf_grid_set_march_segment": [_VP, ctypes.c_int],
+        "sf_grid_set_x_walk": [_VP, ctypes.c_int],
         "sf_run_static": [ctypes.POINTER(GridDesc), _VP, ctypes.c_int, ctypes.c_double,
                           ctypes.c_double, _VP, _VP, ctypes.POINTER(ctypes.c_size_t)],
         "sf_riemann_flux": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -248,6 +249,11 @@ class Grid:
 
     def set_march_segment(self, nodes: int):
         _check(load().sf_grid_set_march_segment(self._h, int(nodes)))
+
+    def set_x_walk(self, mode: int):
+        """x sweep by whole lines (1), by flattened face tiles (0) or the
+        automatic choice (-1)."""
+        _check(load().sf_grid_set_x_walk(self._h, int(mode)))
 
     def profile(self, enable: bool = True):
         _check(load().sf_profile_enable(self._h, int(enable)))

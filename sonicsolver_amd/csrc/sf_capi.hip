@@ -67,6 +67,7 @@ struct sf_grid {
     long long base_seq = 0;
     int base_cur = 0;
     int march_seg = 0;           // nodes per march segment (0: automatic)
+    int x_walk = -1;             // x sweep by lines (1), by flat tiles (0), automatic (-1)
     int cus = 256;               // compute units of the grid's device
     double* dt_hist = nullptr;
     int dt_hist_cap = 0;
@@ -206,7 +207,7 @@ int fill_ghosts(sf_grid* g, double* q, int all, long long tag) {
 // (dz + dy) + dx rather than the reference's (dx + dy) + dz, which changes
 // the last bit at most.
 int run_sweeps(sf_grid* g, const double* q, const StageArgs& A) {
-    SweepCall c{g->G, q, g->S, A, g->stream, g->march_seg, g->cus};
+    SweepCall c{g->G, q, g->S, A, g->stream, g->march_seg, g->cus, g->x_walk};
     int axes[3], na = 0;
     for (int a = 2; a >= 0; --a)
         if (g->G.active[a]) axes[na++] = a;
@@ -807,6 +808,14 @@ int sf_fault_check(sf_grid* g, uint32_t* flags, long long* step, int* phase) {
     if (!g) return fail(SF_E_ARG, "sf_fault_check: null grid");
     if (int rc = set_device(g)) return rc;
     return check_flags(g, "sf_fault_check", flags, step, phase);
+}
+
+int sf_grid_set_x_walk(sf_grid* g, int mode) {
+    if (!g) return fail(SF_E_ARG, "sf_grid_set_x_walk: null grid");
+    if (mode < -1 || mode > 1)
+        return fail(SF_E_ARG, "sf_grid_set_x_walk: -1 (automatic), 0 (flat tiles) or 1 (lines)");
+    g->x_walk = mode;
+    return 0;
 }
 
 int sf_grid_set_march_segment(sf_grid* g, int nodes) {

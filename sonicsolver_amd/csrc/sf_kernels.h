@@ -9,7 +9,10 @@
 //   ghost fill (O(surface))      k_bc_copy4 / k_bc_energy / k_bc_periodic
 //   x sweep                      k_sweep_x:     one lane per x face, the
 //                                               flux difference by a
-//                                               cross-lane shift
+//                                               cross-lane shift; on grids
+//                                               with many x lines
+//                                               k_sweep_xl walks whole
+//                                               lines in 64-node tiles
 //   y / z sweeps                 k_sweep_march: one lane per (x, column
 //                                               segment), marching along the
 //                                               axis with the stencil window

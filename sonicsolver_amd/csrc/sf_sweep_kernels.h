@@ -29,6 +29,14 @@ __device__ __forceinline__ double lane_shift_up1(double x) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// The value of lane 63, as a wave-uniform value.
+__device__ __forceinline__ double lane63(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // The final sweep's operands for node id: its current value, the flux
 // difference the y and z sweeps accumulated and its step-start value.  All
 // three are loaded for every lane, without a test: every lane's node index
@@ -132,6 +140,120 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
             if (TIME != 3 && G.images) write_images(G, A.qout, a, j, k, id, qn);
             dt_local = fmin(dt_local, dl);
         }
+    }
+    raise_fault(flags, A.flags, A.fault_tag, A.tag);
+    if (A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);
+}
+
+// ---------------------------------------------------------------------------
+// x sweep by whole lines, for grids with many x lines: a block (one wave)
+// takes groups of kLineGroup lines and walks each line in tiles of 64
+// nodes, lane t evaluating the upper face of node 64m + t; node 64m + t
+// takes its lower face from lane t-1 by the cross-lane shift, lane 0 from
+// the previous tile's lane 63 (kept in scalar registers), and the first
+// tile of a line from the line's first face, which the group evaluates for
+// its lines at once before walking them.  Rows start on 128-byte lines, so
+// a tile's stage operands and results are whole lines, read and written
+// once.  Every node takes the same faces and the same arithmetic as in
+// k_sweep_x, so the two give identical results.
+// ---------------------------------------------------------------------------
+constexpr int kLineGroup = 16;
+
+template <int SCHEME, int FLUX, int TIME>
+__global__ __launch_bounds__(kXBlock)
+__attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_xl(GridDev G, const double* __restrict__ q,
+                                                      const double* __restrict__ S, StageArgs A) {
+    if (*A.flags) return;   // a fault was raised: keep the step-start state
+    constexpr int NS = Stencil<SCHEME>::ns;
+    __shared__ double first[5][kLineGroup];   // each line's face -1/2
+    const int t = threadIdx.x;
+    const int n0 = G.n[0];
+    const unsigned lines = (unsigned)G.n[1] * (unsigned)G.n[2];
+    const unsigned groups = (lines + kLineGroup - 1) / kLineGroup;
+    const int tiles = (n0 + kXBlock - 1) / kXBlock;
+    unsigned flags = 0;
+    double dt_local = INFINITY;
+    const double lw = (FLUX == 5) ? *A.dt / G.d[0] : 0.0;   // Lax-Wendroff dt / spacing
+    for (unsigned grp = blockIdx.x; grp < groups; grp += gridDim.x) {
+        const unsigned line0 = grp * kLineGroup;
+        // the first face (-1/2) of the group's lines, lane l for line0 + l
+        {
+            const unsigned line = line0 + (unsigned)t;
+            if (t < kLineGroup && line < lines) {
+                const unsigned kz = fast_div(line, G.div_n1);
+                const int j = (int)(line - kz * (unsigned)G.n[1]), k = (int)kz;
+                const long long id = node_index(G, -1, j, k);
+                double st[NS][5];
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + id + (s + 1 - NS / 2)];
+                double f5[5];
+                face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags, lw);
+                if (!finite_sum5(f5)) flags |= kFlagFlux;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) first[v][t] = f5[v];
+            }
+        }
+        __syncthreads();
+        for (int l = 0; l < kLineGroup; ++l) {
+            const unsigned line = line0 + (unsigned)l;
+            if (line >= lines) break;
+            const unsigned kz = fast_div(line, G.div_n1);
+            const int j = (int)(line - kz * (unsigned)G.n[1]), k = (int)kz;
+            double carry[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) carry[v] = first[v][l];
+            for (int m = 0; m < tiles; ++m) {
+                const int a = m * kXBlock + t;   // this lane's node
+                const bool node = a < n0;
+                const long long id = node_index(G, node ? a : n0 - 1, j, k);
+                // stage operands before the face arithmetic (WENO7: after)
+                double pq[5], ps[5], pq0[5];
+                constexpr bool kEarly = SCHEME != 3;
+                if (kEarly) stage_operands<TIME>(G, A, q, S, id, pq, ps, pq0);
+                double f5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                if (node) {
+                    double st[NS][5];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + id + (s + 1 - NS / 2)];
+                    face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags, lw);
+                    if (!finite_sum5(f5)) flags |= kFlagFlux;
+                }
+                double lower[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    const double sh = lane_shift_up1(f5[v]);
+                    lower[v] = t == 0 ? carry[v] : sh;
+                    carry[v] = lane63(f5[v]);   // the next tile's first lower face
+                }
+                if (node) {
+                    if (!kEarly) stage_operands<TIME>(G, A, q, S, id, pq, ps, pq0);
+                    double s5[5];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) {
+                        const double d = f5[v] - lower[v];
+                        s5[v] = ps[v] + d;
+                    }
+                    double dl = INFINITY;
+                    double qn[5];
+                    if (solved_node(G, a, j, k)) {
+                        stage_update_values<TIME>(G, A, id, pq, pq0, s5, flags, dl, qn);
+                    } else if (TIME != 3) {
+#pragma unroll
+                        for (int v = 0; v < 5; ++v) {
+                            qn[v] = pq[v];   // carry_node
+                            A.qout[v * G.P + id] = qn[v];
+                        }
+                    }
+                    if (TIME != 3 && G.images) write_images(G, A.qout, a, j, k, id, qn);
+                    dt_local = fmin(dt_local, dl);
+                }
+            }
+        }
+        __syncthreads();
     }
     raise_fault(flags, A.flags, A.fault_tag, A.tag);
     if (A.reduce_dt) wave_min_atomic(dt_local, A.dtmin);

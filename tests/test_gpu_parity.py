@@ -161,3 +161,36 @@ def test_march_segment_length_does_not_change_the_result(case):
             outs.append(g.download())
     for seg, o in zip((8, 16, 32, 64, 5), outs[1:]):
         assert np.array_equal(o, outs[0]), seg
+
+
+@pytest.mark.parametrize("n,bc,scheme,flux,time", [
+    ((96, 40, 24), ("periodic",) * 3, "weno5", "hllc", "ssprk3"),
+    ((70, 20, 12), ("transmissive", "periodic", "none"), "teno5", "rusanov", "rk4"),
+    ((130, 33, 1), ("none_lo", "transmissive", "transmissive"), "weno7", "roe", "euler"),
+    ((64, 9, 9), ("transmissive",) * 3, "weno3", "steger", "ssprk3"),
+    ((1000, 1, 1), ("transmissive", "transmissive", "transmissive"), "weno5", "laxwendroff",
+     "ssprk3"),
+])
+def test_x_sweep_by_lines_matches_the_oracle_and_the_flat_tiles(n, bc, scheme, flux, time):
+    """The x sweep that walks whole lines in 64-node tiles (the default on
+    grids with many x lines) against the oracle, and bit for bit against the
+    flattened-tile x sweep: every node takes the same faces and the same
+    arithmetic in both.  Lines of 64 to 1000 nodes, whole and partial last
+    tiles, every recipe, zeroGradient, periodic and open ends."""
+    active = tuple(1 if x > 1 else 0 for x in n)
+    q = wl.smooth_field(n, seed=31)
+    kw = dict(n=n, active=active, bc=bc, d=(0.1, 0.12, 0.09), scheme=scheme, flux=flux,
+              time=time)
+    outs = []
+    for mode in (1, 0):
+        print(f"case x walk {mode} {n} {bc} {scheme}/{flux}/{time}", flush=True)
+        with Grid(make_desc(**kw)) as g:
+            g.set_x_walk(mode)
+            g.upload(q)
+            dts = g.run(6, 0.4)
+            outs.append((g.download(), dts))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    ref, ref_dts = ob.run(ob.make_desc(**kw), q, 6, 0.4, 1e300)
+    np.testing.assert_allclose(outs[0][1], ref_dts, rtol=1e-12, atol=0)
+    assert_close(outs[0][0], ref, what=f"x by lines {n} {scheme}/{flux}/{time}")
