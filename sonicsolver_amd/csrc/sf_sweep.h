@@ -57,14 +57,18 @@ int launch_sweep_fm(const SweepCall& c, int axis) {
         } else {
             // Grids with x lines enough for four groups of lines per block
             // slot of the device (three waves per SIMD) walk whole lines
-            // (k_sweep_xl): 512^3 ran its x sweep in 5.66 ms against 5.93
-            // with the flattened tiles, 256^3 (one group per slot and a
-            // third) in 0.81 against 0.78, so smaller grids keep the tiles.
+            // (k_sweep_xl): 512^3 ran its x sweep in 5.43 ms against 5.59
+            // with the flattened tiles (5.66 before the stage operands were
+            // parked in LDS), 256^3 (one group per slot and a third) in 0.81
+            // against 0.78, so smaller grids keep the tiles.
+            // The line walk addresses a lane's stage operands by a 32-bit
+            // byte offset of up to one variable (8 P bytes) past the tile.
             const long long lines = (long long)G.n[1] * G.n[2];
+            const bool fits = (long long)G.P * 8 + 1024 < (1LL << 32);
             const bool by_lines =
-                c.x_walk == 1 ||
-                (c.x_walk < 0 && G.n[0] >= kXBlock &&
-                 lines >= 4LL * kLineGroup * std::max(c.cus, 8) * 12);
+                fits && (c.x_walk == 1 ||
+                         (c.x_walk < 0 && G.n[0] >= kXBlock &&
+                          lines >= 4LL * kLineGroup * std::max(c.cus, 8) * 12));
             if (by_lines) {
                 const long long groups = (lines + kLineGroup - 1) / kLineGroup;
                 const unsigned nb = (unsigned)std::min<long long>(groups, 16384);

@@ -154,26 +154,54 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
 // tile of a line from the line's first face, which the group evaluates for
 // its lines at once before walking them.  Rows start on 128-byte lines, so
 // a tile's stage operands and results are whole lines, read and written
-// once.  Every node takes the same faces and the same arithmetic as in
-// k_sweep_x, so the two give identical results.
+// once; the operands travel into LDS beside the stencil loads, so the face
+// arithmetic runs with no register held for them (four waves per SIMD
+// where it fits 128 registers; 5.43 ms per 512^3 launch against 5.59 with
+// the operands held in registers at three waves).  Every node takes the
+// same faces and the same arithmetic as in k_sweep_x, so the two give
+// identical results.
 // ---------------------------------------------------------------------------
 constexpr int kLineGroup = 16;
 
+// The stage operands of a tile (q, S and q0 of its 64 nodes) are copied
+// into LDS by the memory pipe (16-byte global-to-LDS loads) while the
+// face arithmetic runs, so no register holds them across it.  Slot
+// 6*arr + v holds variable v of array arr (0 q, 1 S, 2 q0); each copy
+// fills two slots, lanes 0-31 the lower and lanes 32-63 the upper, two
+// nodes per lane (slot 6*arr + 5 takes a second copy of variable 4).
+constexpr int kParkSlots = 18;
+
+// Four waves per SIMD where the face arithmetic fits 128 registers without
+// spilling (WENO3; WENO5 with the Rusanov, HLLC, Roe and Lax-Friedrichs
+// fluxes), else three.
+template <int SCHEME, int FLUX>
+constexpr int xl_waves() {
+    return SCHEME == 2 || (SCHEME == 0 && FLUX != 2 && FLUX != 5) ? 4 : 3;
+}
+
 template <int SCHEME, int FLUX, int TIME>
 __global__ __launch_bounds__(kXBlock)
-__attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_xl(GridDev G, const double* __restrict__ q,
-                                                      const double* __restrict__ S, StageArgs A) {
+__attribute__((amdgpu_waves_per_eu(xl_waves<SCHEME, FLUX>(), 8))) void k_sweep_xl(
+    GridDev G, const double* __restrict__ q, const double* __restrict__ S, StageArgs A) {
     if (*A.flags) return;   // a fault was raised: keep the step-start state
     constexpr int NS = Stencil<SCHEME>::ns;
     __shared__ double first[5][kLineGroup];   // each line's face -1/2
+    __shared__ __attribute__((aligned(16))) double park[kParkSlots][kXBlock];
     const int t = threadIdx.x;
     const int n0 = G.n[0];
     const unsigned lines = (unsigned)G.n[1] * (unsigned)G.n[2];
     const unsigned groups = (lines + kLineGroup - 1) / kLineGroup;
     const int tiles = (n0 + kXBlock - 1) / kXBlock;
+    // byte offsets of this lane's two nodes (2i, 2i + 1) from the tile's
+    // first node: in the lower variable of a copy, and in the upper one
+    // (one variable, P doubles, further on); the host keeps 8 P + 1024
+    // below 2^32
+    const int pair = t & 31;
+    const unsigned up = (t >> 5) ? (unsigned)G.P * 8u : 0u;
     unsigned flags = 0;
     double dt_local = INFINITY;
     const double lw = (FLUX == 5) ? *A.dt / G.d[0] : 0.0;   // Lax-Wendroff dt / spacing
+    const double* const arrays[3] = {q, S, A.q0};
     for (unsigned grp = blockIdx.x; grp < groups; grp += gridDim.x) {
         const unsigned line0 = grp * kLineGroup;
         // the first face (-1/2) of the group's lines, lane l for line0 + l
@@ -208,20 +236,35 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_xl(GridDev G, const doub
                 const int a = m * kXBlock + t;   // this lane's node
                 const bool node = a < n0;
                 const long long id = node_index(G, node ? a : n0 - 1, j, k);
-                // stage operands before the face arithmetic (WENO7: after)
-                double pq[5], ps[5], pq0[5];
-                constexpr bool kEarly = SCHEME != 3;
-                if (kEarly) stage_operands<TIME>(G, A, q, S, id, pq, ps, pq0);
-                double f5[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-                if (node) {
-                    double st[NS][5];
+                // the stencil (lanes past the line's end take the last node's)
+                double st[NS][5];
 #pragma unroll
-                    for (int s = 0; s < NS; ++s)
+                for (int s = 0; s < NS; ++s)
 #pragma unroll
-                        for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + id + (s + 1 - NS / 2)];
-                    face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags, lw);
-                    if (!finite_sum5(f5)) flags |= kFlagFlux;
+                    for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + id + (s + 1 - NS / 2)];
+                // the stage operands into LDS; a lane whose nodes lie past
+                // the line's end copies the tile's first two again.  The
+                // barriers keep the loads issued back to back, stencil first,
+                // ahead of the face arithmetic.
+                __builtin_amdgcn_sched_barrier(0);
+                {
+                    const long long id0 = node_index(G, m * kXBlock, j, k);
+                    const int rem = n0 - m * kXBlock;
+                    const unsigned lo = (unsigned)(2 * pair < rem ? pair : 0) * 16u;
+#pragma unroll
+                    for (int c = 0; c < kParkSlots / 2; ++c) {
+                        const int arr = c / 3, v = 2 * (c % 3);
+                        const char* base = (const char*)(arrays[arr] + v * G.P + id0);
+                        const unsigned off = lo + (v < 4 ? up : 0u);
+                        __builtin_amdgcn_global_load_lds(
+                            (__attribute__((address_space(1))) void*)(base + off),
+                            (__attribute__((address_space(3))) void*)&park[2 * c][0], 16, 0, 0);
+                    }
                 }
+                __builtin_amdgcn_sched_barrier(0);
+                double f5[5];
+                face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags, lw);
+                if (!finite_sum5(f5)) flags |= kFlagFlux;
                 double lower[5];
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
@@ -229,8 +272,19 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_xl(GridDev G, const doub
                     lower[v] = t == 0 ? carry[v] : sh;
                     carry[v] = lane63(f5[v]);   // the next tile's first lower face
                 }
+                // the copies have landed once the vector memory count is 0
+                // (the compiler does not count them for the LDS reads)
+                __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+                __builtin_amdgcn_sched_barrier(0);
                 if (node) {
-                    if (!kEarly) stage_operands<TIME>(G, A, q, S, id, pq, ps, pq0);
+                    double pq[5], ps[5], pq0[5];
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) {
+                        pq[v] = park[v][t];
+                        ps[v] = park[6 + v][t];
+                        pq0[v] = park[12 + v][t];
+                    }
                     double s5[5];
 #pragma unroll
                     for (int v = 0; v < 5; ++v) {
