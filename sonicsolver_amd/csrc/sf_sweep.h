@@ -55,12 +55,14 @@ int launch_sweep_fm(const SweepCall& c, int axis) {
         if constexpr (MODE != kFinal) {
             return -1;
         } else {
-            // Grids with x lines enough for four groups of lines per block
-            // slot of the device (three waves per SIMD) walk whole lines
-            // (k_sweep_xl): 512^3 ran its x sweep in 5.43 ms against 5.59
-            // with the flattened tiles (5.66 before the stage operands were
-            // parked in LDS), 256^3 (one group per slot and a third) in 0.81
-            // against 0.78, so smaller grids keep the tiles.
+            // Grids with at least one group of lines per wave slot of the
+            // device at four waves per SIMD walk whole lines (k_sweep_xl):
+            // 512^3 ran its x sweep in 5.43 ms against 5.59 with the
+            // flattened tiles, 256^3 (one group per slot) in 0.75-0.78
+            // against 0.77-0.78 with WENO5, 0.72 against 0.77 with WENO3,
+            // 0.94 against 1.03 with TENO5/HLLC and 1.39 against 1.44 with
+            // WENO7/HLLC (the Steger-Warming sweeps, at three waves, ran 2%
+            // slower); 2D grids and smaller 3D ones keep the tiles.
             // The line walk addresses a lane's stage operands by a 32-bit
             // byte offset of up to one variable (8 P bytes) past the tile.
             const long long lines = (long long)G.n[1] * G.n[2];
@@ -68,7 +70,7 @@ int launch_sweep_fm(const SweepCall& c, int axis) {
             const bool by_lines =
                 fits && (c.x_walk == 1 ||
                          (c.x_walk < 0 && G.n[0] >= kXBlock &&
-                          lines >= 4LL * kLineGroup * std::max(c.cus, 8) * 12));
+                          lines >= 16LL * kLineGroup * std::max(c.cus, 8)));
             if (by_lines) {
                 const long long groups = (lines + kLineGroup - 1) / kLineGroup;
                 const unsigned nb = (unsigned)std::min<long long>(groups, 16384);
