@@ -392,9 +392,11 @@ void set_geometry(sf_grid* g, const sf_grid_desc* desc) {
             // rows start on 128-byte lines: node 0 of every row sits 16
             // doubles into it (its ghosts just before), and the row length
             // is a multiple of 16, so the marches' 64-column blocks read and
-            // write whole cache lines
+            // write whole cache lines; the row holds at least one node past
+            // the last ghost, so the line-walking x sweep can copy the last
+            // ghost as the first of a 16-byte pair
             G.xo = 16;
-            G.m[0] = (G.xo + G.n[0] + G.g[0] + 15) / 16 * 16;
+            G.m[0] = (G.xo + G.n[0] + G.g[0] + 1 + 15) / 16 * 16;
         }
         G.active[a] = desc->active[a];
         G.bc[a] = desc->bc[a];

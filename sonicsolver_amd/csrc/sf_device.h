@@ -239,22 +239,21 @@ __device__ __forceinline__ void carry_node(const GT& G, const StageArgs& A, long
 
 // Stage update of one node from its current value qc and (SSPRK3 after
 // the first stage, RK4) its step-start value q0v; S is the accumulated flux
-// difference.  GT is any descriptor with P, jac, gamma, active[3] and
-// metric[3]; the step's dt is read from *A.dt.  TIME fixes the time recipe
-// (A.time) at compile time; -1 reads it from A.
+// difference and dt the step's dt (*A.dt, which no kernel of a stage
+// changes, so a sweep reads it once).  GT is any descriptor with P, jac,
+// gamma, active[3] and metric[3].  TIME fixes the time recipe (A.time) at
+// compile time; -1 reads it from A.
 template <int TIME = -1, class GT>
-__device__ __forceinline__ void stage_update_values(const GT& G, const StageArgs& A,
-                                                    long long id, const double qc[5],
-                                                    const double q0v[5], const double S[5],
-                                                    unsigned& flags, double& dt_local,
-                                                    double qn[5]) {
+__device__ __forceinline__ void stage_update_dt(const GT& G, const StageArgs& A, long long id,
+                                                const double qc[5], const double q0v[5],
+                                                const double S[5], double dt, unsigned& flags,
+                                                double& dt_local, double qn[5]) {
     const int time = TIME >= 0 ? TIME : A.time;
     if (time == 3) {  // residual only (diagnostics)
 #pragma unroll
         for (int v = 0; v < 5; ++v) A.qout[v * G.P + id] = G.jac * S[v];
         return;
     }
-    const double dt = *A.dt;
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         const double R = G.jac * S[v];
@@ -281,6 +280,18 @@ __device__ __forceinline__ void stage_update_values(const GT& G, const StageArgs
     const double eint = qn[4] - ke;
     if (!(qn[0] > 0.0) || !(eint > 0.0) || !isfinite(eint)) flags |= kFlagState;
     if (A.reduce_dt) dt_local = local_dt(G, qn, A.cfl, flags);
+}
+
+// stage_update_dt with the step's dt read from *A.dt.
+template <int TIME = -1, class GT>
+__device__ __forceinline__ void stage_update_values(const GT& G, const StageArgs& A,
+                                                    long long id, const double qc[5],
+                                                    const double q0v[5], const double S[5],
+                                                    unsigned& flags, double& dt_local,
+                                                    double qn[5]) {
+    const int time = TIME >= 0 ? TIME : A.time;
+    const double dt = time == 3 ? 0.0 : *A.dt;
+    stage_update_dt<TIME>(G, A, id, qc, q0v, S, dt, flags, dt_local, qn);
 }
 
 // Whether a stage of recipe `time` reads the step-start state q0.

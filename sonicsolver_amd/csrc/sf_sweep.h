@@ -55,14 +55,14 @@ int launch_sweep_fm(const SweepCall& c, int axis) {
         if constexpr (MODE != kFinal) {
             return -1;
         } else {
-            // Grids with at least one group of lines per wave slot of the
+            // Grids with a group of 16 lines for every wave slot of the
             // device at four waves per SIMD walk whole lines (k_sweep_xl):
-            // 512^3 ran its x sweep in 5.43 ms against 5.59 with the
-            // flattened tiles, 256^3 (one group per slot) in 0.75-0.78
-            // against 0.77-0.78 with WENO5, 0.72 against 0.77 with WENO3,
-            // 0.94 against 1.03 with TENO5/HLLC and 1.39 against 1.44 with
-            // WENO7/HLLC (the Steger-Warming sweeps, at three waves, ran 2%
-            // slower); 2D grids and smaller 3D ones keep the tiles.
+            // 512^3 ran its x sweep in 5.06 ms against 5.59 with the
+            // flattened tiles, 256^3 in 0.68 against 0.77 with WENO5/HLLC,
+            // 0.67 against 0.77 with WENO5/Steger-Warming, 0.86 against 1.03
+            // with TENO5/HLLC, 1.24 against 1.44 with WENO7/HLLC and 0.72
+            // against 0.77 with WENO3/HLLC; 2D grids and smaller 3D ones,
+            // with too few lines to fill the device, keep the tiles.
             // The line walk addresses a lane's stage operands by a 32-bit
             // byte offset of up to one variable (8 P bytes) past the tile.
             const long long lines = (long long)G.n[1] * G.n[2];

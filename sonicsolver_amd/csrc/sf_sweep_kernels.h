@@ -80,6 +80,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
     const unsigned tiles = (faces - 1 + kXBlock - 2) / (kXBlock - 1);
     unsigned flags = 0;
     double dt_local = INFINITY;
+    const double dt_step = TIME == 3 ? 0.0 : *A.dt;   // the step's dt, read once
     const double lw = (FLUX == 5) ? *A.dt / G.d[0] : 0.0;   // Lax-Wendroff dt / spacing
     // Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
     // each with its own L2; tiles are numbered so that the blocks of one XCD
@@ -129,7 +130,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
             double dl = INFINITY;
             double qn[5];
             if (solved_node(G, a, j, k)) {
-                stage_update_values<TIME>(G, A, id, pq, pq0, s5, flags, dl, qn);
+                stage_update_dt<TIME>(G, A, id, pq, pq0, s5, dt_step, flags, dl, qn);
             } else if (TIME != 3) {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
@@ -153,64 +154,94 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
 // the previous tile's lane 63 (kept in scalar registers), and the first
 // tile of a line from the line's first face, which the group evaluates for
 // its lines at once before walking them.  Rows start on 128-byte lines, so
-// a tile's stage operands and results are whole lines, read and written
-// once; the operands travel into LDS beside the stencil loads, so the face
-// arithmetic runs with no register held for them (four waves per SIMD
-// where it fits 128 registers; 5.43 ms per 512^3 launch against 5.59 with
-// the operands held in registers at three waves).  Every node takes the
-// same faces and the same arithmetic as in k_sweep_x, so the two give
-// identical results.
+// a tile's loads and stores are whole lines.  Every node takes the same
+// faces and the same arithmetic as in k_sweep_x, so the two give identical
+// results.
+//
+// All loads go through LDS by 16-byte global-to-LDS copies, which the
+// memory pipe completes while the wave computes: the conserved variables
+// of the next tile with a halo of four nodes on each side (72 per
+// variable, two buffers used in turn) are requested before the current
+// tile's faces, and the current tile's S and q0 (one 64-node row each,
+// lanes 0-31 one variable and 32-63 the next, two nodes per lane) before
+// its update.  The stencils are read from LDS, so the face arithmetic
+// never waits for memory and no register holds an operand across it.  The
+// copies are not counted by the compiler for LDS reads: one explicit wait
+// for the vector memory count to reach zero, after the faces, covers both
+// the tile's S, q0 and the next tile's variables (all requested before it).
 // ---------------------------------------------------------------------------
 constexpr int kLineGroup = 16;
-
-// The stage operands of a tile (q, S and q0 of its 64 nodes) are copied
-// into LDS by the memory pipe (16-byte global-to-LDS loads) while the
-// face arithmetic runs, so no register holds them across it.  Slot
-// 6*arr + v holds variable v of array arr (0 q, 1 S, 2 q0); each copy
-// fills two slots, lanes 0-31 the lower and lanes 32-63 the upper, two
-// nodes per lane (slot 6*arr + 5 takes a second copy of variable 4).
-constexpr int kParkSlots = 18;
-
-// Four waves per SIMD where the face arithmetic fits 128 registers without
-// spilling (WENO3; WENO5 with the Rusanov, HLLC, Roe and Lax-Friedrichs
-// fluxes), else three.
-template <int SCHEME, int FLUX>
-constexpr int xl_waves() {
-    return SCHEME == 2 || (SCHEME == 0 && FLUX != 2 && FLUX != 5) ? 4 : 3;
-}
+constexpr int kHalo = 4;                      // nodes each side of a tile
+constexpr int kRow = kXBlock + 2 * kHalo;     // 72 doubles per variable
+constexpr int kRowLanes = kRow / 2;           // lanes of 16 bytes per row
 
 template <int SCHEME, int FLUX, int TIME>
 __global__ __launch_bounds__(kXBlock)
-__attribute__((amdgpu_waves_per_eu(xl_waves<SCHEME, FLUX>(), 8))) void k_sweep_xl(
-    GridDev G, const double* __restrict__ q, const double* __restrict__ S, StageArgs A) {
+__attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_xl(GridDev G, const double* __restrict__ q,
+                                                      const double* __restrict__ S, StageArgs A) {
     if (*A.flags) return;   // a fault was raised: keep the step-start state
     constexpr int NS = Stencil<SCHEME>::ns;
+    static_assert(NS / 2 <= kHalo, "the halo covers the stencil");
     __shared__ double first[5][kLineGroup];   // each line's face -1/2
-    __shared__ __attribute__((aligned(16))) double park[kParkSlots][kXBlock];
+    __shared__ __attribute__((aligned(16))) double qb[2][5][kRow];
+    __shared__ __attribute__((aligned(16))) double pk[10][kXBlock];   // S 0-4, q0 5-9
     const int t = threadIdx.x;
     const int n0 = G.n[0];
     const unsigned lines = (unsigned)G.n[1] * (unsigned)G.n[2];
     const unsigned groups = (lines + kLineGroup - 1) / kLineGroup;
     const int tiles = (n0 + kXBlock - 1) / kXBlock;
-    // byte offsets of this lane's two nodes (2i, 2i + 1) from the tile's
-    // first node: in the lower variable of a copy, and in the upper one
-    // (one variable, P doubles, further on); the host keeps 8 P + 1024
-    // below 2^32
+    const int last_needed = n0 - 1 + NS / 2;   // the highest node a stencil reads
+    // S / q0 copies: byte offset of this lane's two nodes in the lower
+    // variable of a copy, and of the upper variable (P doubles further;
+    // the host keeps 8 P + 1024 below 2^32)
     const int pair = t & 31;
     const unsigned up = (t >> 5) ? (unsigned)G.P * 8u : 0u;
     unsigned flags = 0;
     double dt_local = INFINITY;
+    const double dt_step = TIME == 3 ? 0.0 : *A.dt;   // the step's dt, read once
     const double lw = (FLUX == 5) ? *A.dt / G.d[0] : 0.0;   // Lax-Wendroff dt / spacing
-    const double* const arrays[3] = {q, S, A.q0};
+
+    // the 72 nodes 64m-4 .. 64m+67 of line (j, k), all five variables, into
+    // buffer b; lanes 0-35 take two nodes each, a lane whose first node no
+    // stencil reads copies the row's first two again (the second node of a
+    // pair is at most one past the last ghost, which every row holds)
+    auto fetch_q = [&](int b, int m, int j, int k) {
+        if (t < kRowLanes) {
+            const int node0 = m * kXBlock - kHalo;
+            const long long id = node_index(G, node0, j, k);
+            const unsigned off = (node0 + 2 * t <= last_needed ? (unsigned)t : 0u) * 16u;
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                const char* base = (const char*)(q + v * G.P + id);
+                __builtin_amdgcn_global_load_lds(
+                    (__attribute__((address_space(1))) void*)(base + off),
+                    (__attribute__((address_space(3))) void*)&qb[b][v][0], 16, 0, 0);
+            }
+        }
+    };
+    auto wait_copies = [] {
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
     for (unsigned grp = blockIdx.x; grp < groups; grp += gridDim.x) {
         const unsigned line0 = grp * kLineGroup;
+        const unsigned nl = min((unsigned)kLineGroup, lines - line0);
+        int j, k;
+        {
+            const unsigned kz = fast_div(line0, G.div_n1);
+            j = (int)(line0 - kz * (unsigned)G.n[1]);
+            k = (int)kz;
+        }
+        fetch_q(0, 0, j, k);   // the group's first tile
         // the first face (-1/2) of the group's lines, lane l for line0 + l
         {
             const unsigned line = line0 + (unsigned)t;
             if (t < kLineGroup && line < lines) {
                 const unsigned kz = fast_div(line, G.div_n1);
-                const int j = (int)(line - kz * (unsigned)G.n[1]), k = (int)kz;
-                const long long id = node_index(G, -1, j, k);
+                const int jl = (int)(line - kz * (unsigned)G.n[1]), kl = (int)kz;
+                const long long id = node_index(G, -1, jl, kl);
                 double st[NS][5];
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
@@ -223,45 +254,56 @@ __attribute__((amdgpu_waves_per_eu(xl_waves<SCHEME, FLUX>(), 8))) void k_sweep_x
                 for (int v = 0; v < 5; ++v) first[v][t] = f5[v];
             }
         }
+        wait_copies();
         __syncthreads();
-        for (int l = 0; l < kLineGroup; ++l) {
-            const unsigned line = line0 + (unsigned)l;
-            if (line >= lines) break;
-            const unsigned kz = fast_div(line, G.div_n1);
-            const int j = (int)(line - kz * (unsigned)G.n[1]), k = (int)kz;
+        int b = 0;   // the buffer holding the current tile's variables
+        for (unsigned l = 0; l < nl; ++l) {
             double carry[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) carry[v] = first[v][l];
             for (int m = 0; m < tiles; ++m) {
                 const int a = m * kXBlock + t;   // this lane's node
-                const bool node = a < n0;
-                const long long id = node_index(G, node ? a : n0 - 1, j, k);
-                // the stencil (lanes past the line's end take the last node's)
+                const int rem = n0 - m * kXBlock;
+                const bool node = t < rem;
+                const long long id0 = node_index(G, m * kXBlock, j, k);
+                // the next tile's variables (next line's first tile at the
+                // end of a line, none after the group's last)
+                {
+                    int jn = j, kn = k, mn = m + 1;
+                    bool more = true;
+                    if (mn == tiles) {
+                        mn = 0;
+                        more = l + 1 < nl;
+                        if (++jn == G.n[1]) { jn = 0; ++kn; }
+                    }
+                    if (more) fetch_q(b ^ 1, mn, jn, kn);
+                }
+                // this tile's S and q0; a lane whose nodes lie past the
+                // line's end copies the tile's first two again
+                {
+                    const unsigned lo = (unsigned)(2 * pair < rem ? pair : 0) * 16u;
+                    const double* const src[2] = {S, A.q0};
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        const int arr = c / 3, v = 2 * (c % 3);
+                        if (v < 4 || t < 32) {
+                            const char* base = (const char*)(src[arr] + v * G.P + id0);
+                            const unsigned off = lo + (v < 4 ? up : 0u);
+                            __builtin_amdgcn_global_load_lds(
+                                (__attribute__((address_space(1))) void*)(base + off),
+                                (__attribute__((address_space(3))) void*)&pk[5 * arr + v][0], 16, 0,
+                                0);
+                        }
+                    }
+                }
+                // the stencil from LDS (lanes past the line's end take the
+                // last node's)
+                const int ti = node ? t : rem - 1;
                 double st[NS][5];
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
 #pragma unroll
-                    for (int v = 0; v < 5; ++v) st[s][v] = q[v * G.P + id + (s + 1 - NS / 2)];
-                // the stage operands into LDS; a lane whose nodes lie past
-                // the line's end copies the tile's first two again.  The
-                // barriers keep the loads issued back to back, stencil first,
-                // ahead of the face arithmetic.
-                __builtin_amdgcn_sched_barrier(0);
-                {
-                    const long long id0 = node_index(G, m * kXBlock, j, k);
-                    const int rem = n0 - m * kXBlock;
-                    const unsigned lo = (unsigned)(2 * pair < rem ? pair : 0) * 16u;
-#pragma unroll
-                    for (int c = 0; c < kParkSlots / 2; ++c) {
-                        const int arr = c / 3, v = 2 * (c % 3);
-                        const char* base = (const char*)(arrays[arr] + v * G.P + id0);
-                        const unsigned off = lo + (v < 4 ? up : 0u);
-                        __builtin_amdgcn_global_load_lds(
-                            (__attribute__((address_space(1))) void*)(base + off),
-                            (__attribute__((address_space(3))) void*)&park[2 * c][0], 16, 0, 0);
-                    }
-                }
-                __builtin_amdgcn_sched_barrier(0);
+                    for (int v = 0; v < 5; ++v) st[s][v] = qb[b][v][ti + kHalo + s + 1 - NS / 2];
                 double f5[5];
                 face_flux<0, SCHEME, FLUX>(st, G.gamma, G.area[0], f5, flags, lw);
                 if (!finite_sum5(f5)) flags |= kFlagFlux;
@@ -272,18 +314,15 @@ __attribute__((amdgpu_waves_per_eu(xl_waves<SCHEME, FLUX>(), 8))) void k_sweep_x
                     lower[v] = t == 0 ? carry[v] : sh;
                     carry[v] = lane63(f5[v]);   // the next tile's first lower face
                 }
-                // the copies have landed once the vector memory count is 0
-                // (the compiler does not count them for the LDS reads)
-                __builtin_amdgcn_sched_barrier(0);
-                __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
-                __builtin_amdgcn_sched_barrier(0);
+                wait_copies();
                 if (node) {
+                    const long long id = id0 + t;
                     double pq[5], ps[5], pq0[5];
 #pragma unroll
                     for (int v = 0; v < 5; ++v) {
-                        pq[v] = park[v][t];
-                        ps[v] = park[6 + v][t];
-                        pq0[v] = park[12 + v][t];
+                        pq[v] = qb[b][v][t + kHalo];
+                        ps[v] = pk[v][t];
+                        pq0[v] = pk[5 + v][t];
                     }
                     double s5[5];
 #pragma unroll
@@ -294,7 +333,7 @@ __attribute__((amdgpu_waves_per_eu(xl_waves<SCHEME, FLUX>(), 8))) void k_sweep_x
                     double dl = INFINITY;
                     double qn[5];
                     if (solved_node(G, a, j, k)) {
-                        stage_update_values<TIME>(G, A, id, pq, pq0, s5, flags, dl, qn);
+                        stage_update_dt<TIME>(G, A, id, pq, pq0, s5, dt_step, flags, dl, qn);
                     } else if (TIME != 3) {
 #pragma unroll
                         for (int v = 0; v < 5; ++v) {
@@ -305,7 +344,9 @@ __attribute__((amdgpu_waves_per_eu(xl_waves<SCHEME, FLUX>(), 8))) void k_sweep_x
                     if (TIME != 3 && G.images) write_images(G, A.qout, a, j, k, id, qn);
                     dt_local = fmin(dt_local, dl);
                 }
+                b ^= 1;
             }
+            if (++j == G.n[1]) { j = 0; ++k; }
         }
         __syncthreads();
     }
