@@ -147,6 +147,10 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
         const int4 T = B.xtiles[tile];
         if (!step_live(B, T.x)) continue;  // uniform over the block
         const BatchItemDev it = B.items[T.x];
+        // the problem's dt, fixed during a stage: read before the face
+        // arithmetic, whose duration hides the load (after the tile's
+        // stores, which might alias it, it would be read again)
+        const double dt_item = B.dt[T.x];
         const int per_line = it.n0 + 1;
         const int f = T.y + t;
         const bool face = f < per_line * it.n1;
@@ -200,7 +204,7 @@ __device__ __forceinline__ void x_tiles(const BatchDev& B, const double* __restr
             const ItemView G = item_view(B, it);
             double qn[5];
             if (item_solved(it, a, j)) {
-                stage_update_values<SF_TIME_SSPRK3>(G, A, id, pq, pq0, s5, flags, dl, qn);
+                stage_update_dt<SF_TIME_SSPRK3>(G, A, id, pq, pq0, s5, dt_item, flags, dl, qn);
             } else {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
