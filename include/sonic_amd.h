@@ -207,9 +207,9 @@ int sf_fault_check(sf_grid *grid, uint32_t *flags, long long *step, int *phase);
 int sf_grid_set_march_segment(sf_grid *grid, int nodes);
 /* How the x sweep walks the grid: 1 by whole x lines in 64-node tiles,
  * 0 by tiles of 63 faces over all lines numbered as one sequence, -1 (the
- * default) by lines where nx >= 64 and the grid has a group of 16 x lines
- * for every wave slot of the device at four waves per SIMD (65536 lines
- * on a 256-CU device: 256^3 and up), else by flattened tiles.  Grids whose
+ * default) by lines where nx >= 64 and the grid has at least 256 x lines
+ * per compute unit (65536 lines on a 256-CU device: 256^3 and up), else by
+ * flattened tiles.  Grids whose
  * variables exceed 2^32 - 1024 bytes always take the flattened tiles.  No
  * reference counterpart; results do not depend on it. */
 int sf_grid_set_x_walk(sf_grid *grid, int mode);

@@ -55,14 +55,15 @@ int launch_sweep_fm(const SweepCall& c, int axis) {
         if constexpr (MODE != kFinal) {
             return -1;
         } else {
-            // Grids with a group of 16 lines for every wave slot of the
-            // device at four waves per SIMD walk whole lines (k_sweep_xl):
-            // 512^3 ran its x sweep in 5.06 ms against 5.59 with the
-            // flattened tiles, 256^3 in 0.68 against 0.77 with WENO5/HLLC,
-            // 0.67 against 0.77 with WENO5/Steger-Warming, 0.86 against 1.03
-            // with TENO5/HLLC, 1.24 against 1.44 with WENO7/HLLC and 0.72
-            // against 0.77 with WENO3/HLLC; 2D grids and smaller 3D ones,
-            // with too few lines to fill the device, keep the tiles.
+            // Grids with at least 256 x lines per CU (16 lines for every
+            // wave slot at four waves per SIMD) walk whole lines
+            // (k_sweep_xl): 512^3 ran its x sweep in 5.03 ms against 5.59
+            // with the flattened tiles, 256^3 in 0.67 against 0.77 with
+            // WENO5/HLLC, 0.67 against 0.77 with WENO5/Steger-Warming, 0.82
+            // against 1.03 with TENO5/HLLC, 1.21 against 1.44 with
+            // WENO7/HLLC and 0.72 against 0.77 with WENO3/HLLC; 2D grids and
+            // smaller 3D ones, with too few lines to fill the device, keep
+            // the tiles.
             // The line walk addresses a lane's stage operands by a 32-bit
             // byte offset of up to one variable (8 P bytes) past the tile.
             const long long lines = (long long)G.n[1] * G.n[2];
@@ -70,7 +71,7 @@ int launch_sweep_fm(const SweepCall& c, int axis) {
             const bool by_lines =
                 fits && (c.x_walk == 1 ||
                          (c.x_walk < 0 && G.n[0] >= kXBlock &&
-                          lines >= 16LL * kLineGroup * std::max(c.cus, 8)));
+                          lines >= 256LL * std::max(c.cus, 8)));
             if (by_lines) {
                 const long long groups = (lines + kLineGroup - 1) / kLineGroup;
                 const unsigned nb = (unsigned)std::min<long long>(groups, 16384);

@@ -170,7 +170,7 @@ __attribute__((amdgpu_waves_per_eu(3, 8))) void k_sweep_x(GridDev G, const doubl
 // for the vector memory count to reach zero, after the faces, covers both
 // the tile's S, q0 and the next tile's variables (all requested before it).
 // ---------------------------------------------------------------------------
-constexpr int kLineGroup = 16;
+constexpr int kLineGroup = 8;   // 16 ran 1-2.5% slower on 256^3, the same on 512^3
 constexpr int kHalo = 4;                      // nodes each side of a tile
 constexpr int kRow = kXBlock + 2 * kHalo;     // 72 doubles per variable
 constexpr int kRowLanes = kRow / 2;           // lanes of 16 bytes per row
